@@ -1,0 +1,281 @@
+// C ABI of libsgrid.so (include/sgrid.h): argument checks, handle ownership and
+// dispatch to the host tables (sg_grid.cpp) and the HIP launchers (sg_kernels.hip).
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "../../include/sgrid.h"
+#include "sg_grid.hpp"
+#include "sg_kernels.hpp"
+
+// Device tables are uploaded on the first device call, to the device current then, so
+// that creating a grid and the host bijection need no GPU.
+struct sg_grid {
+    sg::Grid g;
+    sg::DeviceTables t;
+    sg::EvalTuning tune;
+    int hier_tile_values = 1024;
+    std::mutex mu;
+    bool uploaded = false;
+};
+
+namespace {
+
+int fail(int st, const std::string& msg) {
+    sg::set_error(msg);
+    return st;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Every device entry point runs on the device that holds the grid's tables; the first
+// one uploads them.
+int check_device(const sg_grid* hc) {
+    sg_grid* h = const_cast<sg_grid*>(hc);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return fail(sg::kHip, "hipGetDevice failed");
+    std::lock_guard<std::mutex> lock(h->mu);
+    if (!h->uploaded) {
+        int st = sg::upload_tables(h->g, h->t);
+        if (st != sg::kOk) {
+            sg::free_tables(h->t);
+            return st;
+        }
+        h->uploaded = true;
+    }
+    if (cur != h->t.device)
+        return fail(sg::kInvalid, "current device differs from the grid's device");
+    return sg::kOk;
+}
+
+int finish_create(sg_grid* h, sg_grid** out) {
+    int st = sg::build_launch_tables(h->g, h->hier_tile_values);
+    if (st != sg::kOk) {
+        delete h;
+        return st;
+    }
+    *out = h;
+    return sg::kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sg_last_error(void) { return sg::last_error().c_str(); }
+const char* sg_status_string(int status) { return sg::status_name(status); }
+int sg_version(void) { return 100; }
+
+int sg_grid_create(int num_dims, int num_levels, const int* limits, sg_grid** out) {
+    if (!out) return fail(sg::kInvalid, "sg_grid_create: out is NULL");
+    *out = nullptr;
+    sg_grid* h = new (std::nothrow) sg_grid();
+    if (!h) return fail(sg::kNoMem, "sg_grid_create: host allocation failed");
+    int st = sg::build_regular(h->g, num_dims, num_levels, limits);
+    if (st != sg::kOk) {
+        delete h;
+        return st;
+    }
+    return finish_create(h, out);
+}
+
+int sg_grid_create_explicit(int num_dims, const unsigned char* levels, long long count,
+                            sg_grid** out) {
+    if (!out || !levels) return fail(sg::kInvalid, "sg_grid_create_explicit: NULL argument");
+    *out = nullptr;
+    sg_grid* h = new (std::nothrow) sg_grid();
+    if (!h) return fail(sg::kNoMem, "sg_grid_create_explicit: host allocation failed");
+    int st = sg::build_explicit(h->g, num_dims, levels, count);
+    if (st != sg::kOk) {
+        delete h;
+        return st;
+    }
+    return finish_create(h, out);
+}
+
+int sg_grid_destroy(sg_grid* grid) {
+    if (!grid) return sg::kOk;
+    if (grid->uploaded) sg::free_tables(grid->t);
+    delete grid;
+    return sg::kOk;
+}
+
+int sg_grid_info(const sg_grid* grid, int* num_dims, int* num_levels,
+                 long long* num_grid_points, long long* num_subspaces) {
+    if (!grid) return fail(sg::kInvalid, "sg_grid_info: NULL grid");
+    if (num_dims) *num_dims = grid->g.d;
+    if (num_levels) *num_levels = grid->g.n;
+    if (num_grid_points) *num_grid_points = grid->g.N;
+    if (num_subspaces) *num_subspaces = grid->g.S;
+    return sg::kOk;
+}
+
+int sg_grid_tables(const sg_grid* grid, long long* dp, long long* lidx) {
+    if (!grid) return fail(sg::kInvalid, "sg_grid_tables: NULL grid");
+    if (dp) {
+        if (grid->g.explicit_set)
+            return fail(sg::kUnsupported, "sg_grid_tables: explicit level sets have no dp matrix");
+        for (size_t i = 0; i < grid->g.dp.size(); ++i) dp[i] = grid->g.dp[i];
+    }
+    if (lidx)
+        for (size_t i = 0; i < grid->g.lidx.size(); ++i) lidx[i] = grid->g.lidx[i];
+    return sg::kOk;
+}
+
+int sg_grid_subspaces(const sg_grid* grid, unsigned char* levels, long long* base) {
+    if (!grid) return fail(sg::kInvalid, "sg_grid_subspaces: NULL grid");
+    if (levels) std::memcpy(levels, grid->g.sub_levels.data(), grid->g.sub_levels.size());
+    if (base)
+        for (int64_t i = 0; i < grid->g.S; ++i) base[i] = grid->g.sub_base[i];
+    return sg::kOk;
+}
+
+int sg_grid_set_tuning(sg_grid* grid, int eval_block, int hier_tile_values) {
+    if (!grid) return fail(sg::kInvalid, "sg_grid_set_tuning: NULL grid");
+    if (eval_block != 0 && eval_block != 64 && eval_block != 128 && eval_block != 256)
+        return fail(sg::kInvalid, "sg_grid_set_tuning: eval_block must be 64, 128 or 256");
+    if (hier_tile_values != 0 && hier_tile_values < 64)
+        return fail(sg::kInvalid, "sg_grid_set_tuning: hier_tile_values must be >= 64");
+    std::lock_guard<std::mutex> lock(grid->mu);
+    if (eval_block) grid->tune.block = eval_block;
+    if (hier_tile_values && hier_tile_values != grid->hier_tile_values) {
+        grid->hier_tile_values = hier_tile_values;
+        int st = sg::build_launch_tables(grid->g, hier_tile_values);
+        if (st != sg::kOk) return st;
+        if (grid->uploaded) {            // re-upload on the next device call
+            sg::free_tables(grid->t);
+            grid->uploaded = false;
+        }
+    }
+    return sg::kOk;
+}
+
+int sg_gp2idx(const sg_grid* grid, const int* levels, const int* indices, long long* index) {
+    if (!grid || !levels || !indices || !index) return fail(sg::kInvalid, "sg_gp2idx: NULL argument");
+    int64_t o = sg::gp2idx(grid->g, levels, indices);
+    if (o < 0) return fail(sg::kRange, "sg_gp2idx: point is not in the grid");
+    *index = o;
+    return sg::kOk;
+}
+
+int sg_idx2gp(const sg_grid* grid, long long index, int* levels, int* indices) {
+    if (!grid || !levels || !indices) return fail(sg::kInvalid, "sg_idx2gp: NULL argument");
+    int st = sg::idx2gp(grid->g, index, levels, indices);
+    if (st != sg::kOk) return fail(st, "sg_idx2gp: offset out of range");
+    return sg::kOk;
+}
+
+int sg_idx2l(const sg_grid* grid, long long rank, int level_sum, int* levels) {
+    if (!grid || !levels) return fail(sg::kInvalid, "sg_idx2l: NULL argument");
+    int st = sg::idx2l(grid->g, rank, level_sum, levels);
+    if (st != sg::kOk) return fail(st, "sg_idx2l: rank or level sum out of range");
+    return sg::kOk;
+}
+
+int sg_l2idx(const sg_grid* grid, const int* levels, long long* rank) {
+    if (!grid || !levels || !rank) return fail(sg::kInvalid, "sg_l2idx: NULL argument");
+    if (sg::subspace_id(grid->g, levels) < 0)
+        return fail(sg::kRange, "sg_l2idx: level vector is not in the grid");
+    *rank = sg::l2idx(grid->g, levels);
+    return sg::kOk;
+}
+
+int sg_idx2gp_dev(const sg_grid* grid, const int* d_index, long long count, int* d_levels,
+                  int* d_indices, void* stream) {
+    if (!grid || count < 0 || (count > 0 && (!d_levels || !d_indices)))
+        return fail(sg::kInvalid, "sg_idx2gp_dev: bad argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_idx2gp(grid->g, grid->t, d_index, count, d_levels, d_indices,
+                             as_stream(stream));
+}
+
+int sg_gp2idx_dev(const sg_grid* grid, const int* d_levels, const int* d_indices,
+                  long long count, int* d_index, void* stream) {
+    if (!grid || count < 0 || (count > 0 && (!d_levels || !d_indices || !d_index)))
+        return fail(sg::kInvalid, "sg_gp2idx_dev: bad argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_gp2idx(grid->g, grid->t, d_levels, d_indices, count, d_index,
+                             as_stream(stream));
+}
+
+int sg_grid_coords_f64(const sg_grid* grid, double* d_coords, void* stream) {
+    if (!grid || !d_coords) return fail(sg::kInvalid, "sg_grid_coords_f64: NULL argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_grid_coords_f64(grid->g, grid->t, d_coords, as_stream(stream));
+}
+
+int sg_hierarchize_f64(const sg_grid* grid, double* d_values, void* stream) {
+    if (!grid || !d_values) return fail(sg::kInvalid, "sg_hierarchize_f64: NULL argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_hierarchize_f64(grid->g, grid->t, d_values, as_stream(stream));
+}
+
+int sg_hierarchize_f32(const sg_grid* grid, float* d_values, void* stream) {
+    if (!grid || !d_values) return fail(sg::kInvalid, "sg_hierarchize_f32: NULL argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_hierarchize_f32(grid->g, grid->t, d_values, as_stream(stream));
+}
+
+#define SG_EVAL_CHECK(name)                                                            \
+    if (!grid || num_evals < 0 || (num_evals > 0 && (!d_alpha || !d_coords || !d_out))) \
+        return fail(sg::kInvalid, name ": bad argument");                              \
+    {                                                                                  \
+        int st_ = check_device(grid);                                                  \
+        if (st_ != sg::kOk) return st_;                                                \
+    }
+
+int sg_evaluate_f64(const sg_grid* grid, const double* d_alpha, const double* d_coords,
+                    long long num_evals, double* d_out, void* stream) {
+    SG_EVAL_CHECK("sg_evaluate_f64")
+    return sg::launch_evaluate_f64(grid->g, grid->t, d_alpha, d_coords, num_evals, d_out,
+                                   grid->tune, as_stream(stream));
+}
+
+int sg_evaluate_f32c(const sg_grid* grid, const float* d_alpha, const float* d_coords,
+                     long long num_evals, float* d_out, void* stream) {
+    SG_EVAL_CHECK("sg_evaluate_f32c")
+    return sg::launch_evaluate_f32c(grid->g, grid->t, d_alpha, d_coords, num_evals, d_out,
+                                    grid->tune, as_stream(stream));
+}
+
+int sg_evaluate_ref_f64(const sg_grid* grid, const double* d_alpha, const double* d_coords,
+                        long long num_evals, double* d_out, void* stream) {
+    SG_EVAL_CHECK("sg_evaluate_ref_f64")
+    return sg::launch_evaluate_ref_f64(grid->g, grid->t, d_alpha, d_coords, num_evals, d_out,
+                                       as_stream(stream));
+}
+
+int sg_evaluate_ref_f32(const sg_grid* grid, const float* d_alpha, const float* d_coords,
+                        long long num_evals, float* d_out, void* stream) {
+    SG_EVAL_CHECK("sg_evaluate_ref_f32")
+    return sg::launch_evaluate_ref_f32(grid->g, grid->t, d_alpha, d_coords, num_evals, d_out,
+                                       as_stream(stream));
+}
+
+int sg_uniform_points_f64(long long count, unsigned long long seed, double* d_out,
+                          void* stream) {
+    if (count < 0 || (count > 0 && !d_out)) return fail(sg::kInvalid, "sg_uniform_points_f64: bad argument");
+    return sg::launch_uniform_points_f64(count, seed, d_out, as_stream(stream));
+}
+
+int sg_uniform_points_f32(long long count, unsigned long long seed, float* d_out,
+                          void* stream) {
+    if (count < 0 || (count > 0 && !d_out)) return fail(sg::kInvalid, "sg_uniform_points_f32: bad argument");
+    return sg::launch_uniform_points_f32(count, seed, d_out, as_stream(stream));
+}
+
+int sg_convert_f64_to_f32(const double* d_in, float* d_out, long long count, void* stream) {
+    if (count < 0 || (count > 0 && (!d_in || !d_out)))
+        return fail(sg::kInvalid, "sg_convert_f64_to_f32: bad argument");
+    return sg::launch_f64_to_f32(d_in, d_out, count, as_stream(stream));
+}
+
+}  // extern "C"

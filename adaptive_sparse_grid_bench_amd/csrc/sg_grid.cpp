@@ -1,0 +1,385 @@
+// Host-side grid construction, bijection and launch tables; see sg_grid.hpp.
+#include "sg_grid.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace sg {
+
+namespace {
+thread_local std::string g_last_error;
+
+uint64_t level_key(const int* l, int d) {
+    uint64_t k = 0;
+    for (int j = 0; j < d; ++j) k = k * 31u + static_cast<uint64_t>(l[j]) + 1u;
+    return k;
+}
+uint64_t level_key(const uint8_t* l, int d) {
+    uint64_t k = 0;
+    for (int j = 0; j < d; ++j) k = k * 31u + static_cast<uint64_t>(l[j]) + 1u;
+    return k;
+}
+
+// compact-order key: level sum, then l_{d-1}, l_{d-2}, ... ascending (asg.c:266-279)
+bool compact_less(const uint8_t* a, const uint8_t* b, int d) {
+    int sa = 0, sb = 0;
+    for (int j = 0; j < d; ++j) { sa += a[j]; sb += b[j]; }
+    if (sa != sb) return sa < sb;
+    for (int j = d - 1; j >= 0; --j)
+        if (a[j] != b[j]) return a[j] < b[j];
+    return false;
+}
+
+void fill_offsets(Grid& g) {
+    g.lidx.assign(g.n + 1, 0);
+    g.cnt.assign(g.n + 1, 0);
+    g.sub_base.resize(g.S);
+    int64_t off = 0;
+    int64_t q = 0;
+    for (int s = 0; s < g.n; ++s) {
+        g.lidx[s] = off;
+        g.cnt[s] = q;
+        while (q < g.S) {
+            int sum = 0;
+            for (int j = 0; j < g.d; ++j) sum += g.sub_levels[q * g.d + j];
+            if (sum != s) break;
+            g.sub_base[q] = static_cast<int32_t>(off);
+            off += int64_t(1) << s;
+            ++q;
+        }
+    }
+    g.lidx[g.n] = off;
+    g.cnt[g.n] = q;
+    g.N = off;
+    for (int j = 0; j < g.d; ++j) g.max_level[j] = 0;
+    for (int64_t i = 0; i < g.S; ++i)
+        for (int j = 0; j < g.d; ++j)
+            g.max_level[j] = std::max<int>(g.max_level[j], g.sub_levels[i * g.d + j]);
+}
+}  // namespace
+
+const char* status_name(int st) {
+    switch (st) {
+        case kOk: return "ok";
+        case kInvalid: return "invalid argument";
+        case kNoMem: return "out of memory";
+        case kHip: return "HIP runtime error";
+        case kRange: return "out of range";
+        case kUnsupported: return "unsupported shape";
+        default: return "unknown status";
+    }
+}
+void set_error(const std::string& msg) { g_last_error = msg; }
+const std::string& last_error() { return g_last_error; }
+
+// dp[i][s] = number of level vectors with i+1 components, sum s, l_j <= limits[j]
+// (asg.c:108-129, in the direct form of asg.cu:79-89; the s > limits[0] entries of
+// row 0 are 0 where asg.c leaves them unset).
+int build_regular(Grid& g, int d, int n, const int* limits) {
+    if (d < 1 || d > kMaxDims || n < 1 || n > kMaxLevelSum + 1) {
+        set_error("build_regular: need 1 <= d <= 20 and 1 <= n <= 27");
+        return kInvalid;
+    }
+    g = Grid();
+    g.d = d;
+    g.n = n;
+    for (int j = 0; j < d; ++j) {
+        int lim = limits ? limits[j] : n - 1;
+        if (lim < 0) {
+            set_error("build_regular: negative limit");
+            return kInvalid;
+        }
+        g.limits[j] = std::min(lim, n - 1);
+        if (g.limits[j] > kMaxLevel) {
+            set_error("build_regular: a per-dimension level above 15 is not supported");
+            return kUnsupported;
+        }
+    }
+    g.dp.assign(size_t(d) * n, 0);
+    for (int s = 0; s < n; ++s) g.dp[s] = (s <= g.limits[0]) ? 1 : 0;
+    for (int i = 1; i < d; ++i)
+        for (int s = 0; s < n; ++s) {
+            int64_t acc = 0;
+            for (int k = 0; k <= s && k <= g.limits[i]; ++k) acc += g.dp[(i - 1) * n + (s - k)];
+            g.dp[size_t(i) * n + s] = acc;
+        }
+    int64_t S = 0, N = 0;
+    for (int s = 0; s < n; ++s) {
+        S += g.dp[size_t(d - 1) * n + s];
+        N += g.dp[size_t(d - 1) * n + s] << s;
+    }
+    if (N >= (int64_t(1) << 31)) {
+        set_error("build_regular: grid has 2^31 points or more (offsets are int32)");
+        return kRange;
+    }
+    g.S = S;
+    g.sub_levels.resize(size_t(S) * d);
+    int64_t q = 0;
+    int lv[kMaxDims];
+    for (int s = 0; s < n; ++s)
+        for (int64_t r = 0; r < g.dp[size_t(d - 1) * n + s]; ++r, ++q) {
+            idx2l(g, r, s, lv);
+            for (int j = 0; j < d; ++j) g.sub_levels[q * d + j] = static_cast<uint8_t>(lv[j]);
+        }
+    fill_offsets(g);
+    return kOk;
+}
+
+int build_explicit(Grid& g, int d, const uint8_t* levels, int64_t count) {
+    if (d < 1 || d > kMaxDims || count < 1) {
+        set_error("build_explicit: need 1 <= d <= 20 and a non-empty level set");
+        return kInvalid;
+    }
+    g = Grid();
+    g.d = d;
+    g.explicit_set = true;
+    std::vector<int64_t> order(count);
+    for (int64_t i = 0; i < count; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return compact_less(levels + a * d, levels + b * d, d);
+    });
+    int maxsum = 0;
+    g.sub_levels.clear();
+    for (int64_t i = 0; i < count; ++i) {
+        const uint8_t* l = levels + order[i] * d;
+        if (i > 0 && std::memcmp(l, levels + order[i - 1] * d, d) == 0) continue;  // dedupe
+        int sum = 0;
+        for (int j = 0; j < d; ++j) {
+            if (l[j] > kMaxLevel) {
+                set_error("build_explicit: a per-dimension level above 15 is not supported");
+                return kUnsupported;
+            }
+            sum += l[j];
+        }
+        maxsum = std::max(maxsum, sum);
+        g.sub_levels.insert(g.sub_levels.end(), l, l + d);
+    }
+    if (maxsum > kMaxLevelSum) {
+        set_error("build_explicit: level sum above 26 is not supported");
+        return kUnsupported;
+    }
+    g.S = int64_t(g.sub_levels.size()) / d;
+    g.n = maxsum + 1;
+    for (int64_t i = 0; i < g.S; ++i) g.index_of[level_key(&g.sub_levels[i * d], d)] = i;
+    // downward closed: every l - e_k (l_k > 0) must be present
+    int lv[kMaxDims];
+    for (int64_t i = 0; i < g.S; ++i) {
+        for (int j = 0; j < d; ++j) lv[j] = g.sub_levels[i * d + j];
+        for (int j = 0; j < d; ++j) {
+            if (lv[j] == 0) continue;
+            --lv[j];
+            bool ok = g.index_of.count(level_key(lv, d)) != 0;
+            ++lv[j];
+            if (!ok) {
+                set_error("build_explicit: level set is not downward closed");
+                return kInvalid;
+            }
+        }
+    }
+    for (int j = 0; j < d; ++j) g.limits[j] = kMaxLevel;
+    fill_offsets(g);
+    for (int j = 0; j < d; ++j) g.limits[j] = g.max_level[j];
+    if (g.N >= (int64_t(1) << 31)) {
+        set_error("build_explicit: grid has 2^31 points or more (offsets are int32)");
+        return kRange;
+    }
+    return kOk;
+}
+
+int64_t subspace_id(const Grid& g, const int* levels) {
+    int sum = 0;
+    for (int j = 0; j < g.d; ++j) {
+        if (levels[j] < 0 || levels[j] > g.limits[j]) return -1;
+        sum += levels[j];
+    }
+    if (sum >= g.n) return -1;
+    if (g.explicit_set) {
+        auto it = g.index_of.find(level_key(levels, g.d));
+        return it == g.index_of.end() ? -1 : it->second;
+    }
+    return g.cnt[sum] + l2idx(g, levels);
+}
+
+// rank of a level vector within its level sum: asg.c:266-279
+int64_t l2idx(const Grid& g, const int* levels) {
+    if (g.explicit_set) {
+        int64_t id = subspace_id(g, levels);
+        if (id < 0) return -1;
+        int sum = 0;
+        for (int j = 0; j < g.d; ++j) sum += levels[j];
+        return id - g.cnt[sum];
+    }
+    int64_t rank = 0;
+    int sum = levels[0];
+    for (int i = 1; i < g.d; ++i) {
+        sum += levels[i];
+        for (int j = 0; j < levels[i]; ++j) rank += g.dp[size_t(i - 1) * g.n + (sum - j)];
+    }
+    return rank;
+}
+
+// level vector of the given rank within level sum `sum`: asg.c:245-263
+int idx2l(const Grid& g, int64_t rank, int sum, int* levels) {
+    if (sum < 0 || sum >= g.n || rank < 0) return kRange;
+    if (g.explicit_set) {
+        int64_t id = g.cnt[sum] + rank;
+        if (id >= g.cnt[sum + 1]) return kRange;
+        for (int j = 0; j < g.d; ++j) levels[j] = g.sub_levels[id * g.d + j];
+        return kOk;
+    }
+    if (rank >= g.dp[size_t(g.d - 1) * g.n + sum]) return kRange;
+    for (int i = g.d - 1; i >= 1; --i) {
+        int l = 0;
+        while (rank >= g.dp[size_t(i - 1) * g.n + (sum - l)]) {
+            rank -= g.dp[size_t(i - 1) * g.n + (sum - l)];
+            ++l;
+        }
+        levels[i] = l;
+        sum -= l;
+    }
+    levels[0] = sum;
+    return kOk;
+}
+
+// (levels, indices) -> offset: asg.c:187-207
+int64_t gp2idx(const Grid& g, const int* levels, const int* indices) {
+    int64_t id = subspace_id(g, levels);
+    if (id < 0) return -1;
+    int64_t in_block = 0;
+    for (int j = 0; j < g.d; ++j) {
+        if (indices[j] < 0 || indices[j] >= (1 << levels[j])) return -1;
+        in_block = (in_block << levels[j]) + indices[j];
+    }
+    return int64_t(g.sub_base[id]) + in_block;
+}
+
+// offset -> (levels, indices): asg.c:210-243
+int idx2gp(const Grid& g, int64_t offset, int* levels, int* indices) {
+    if (offset < 0 || offset >= g.N) return kRange;
+    int s = 0;
+    while (s + 1 < g.n && offset >= g.lidx[s + 1]) ++s;
+    int64_t r = offset - g.lidx[s];
+    int64_t rank = r >> s;
+    int64_t bits = r & ((int64_t(1) << s) - 1);
+    int st = idx2l(g, rank, s, levels);
+    if (st != kOk) return st;
+    for (int j = g.d - 1; j >= 0; --j) {
+        indices[j] = static_cast<int>(bits & ((int64_t(1) << levels[j]) - 1));
+        bits >>= levels[j];
+    }
+    return kOk;
+}
+
+namespace {
+
+// Depth-first walk over the level set with dimension d-1 outermost and dimension 0
+// innermost; visit(levels) is called at every leaf.  For a fixed level sum the leaves
+// come in rank order, because rank order is lexicographic from dimension d-1 down.
+template <class F>
+void dfs_walk(const Grid& g, F&& visit, std::vector<uint8_t>* bounds) {
+    int lv[kMaxDims] = {};
+    int d = g.d;
+    // largest admissible l_k given the levels already fixed above k (and zeros below)
+    auto bound_at = [&](int k, int rem) {
+        if (!g.explicit_set) return std::min(rem, g.limits[k]);
+        int b = -1;
+        for (int l = 0; l <= std::min(rem, g.limits[k]); ++l) {
+            lv[k] = l;
+            for (int j = 0; j < k; ++j) lv[j] = 0;
+            if (subspace_id(g, lv) < 0) break;
+            b = l;
+        }
+        lv[k] = 0;
+        return b;
+    };
+    struct Rec {
+        static void go(const Grid& g, int k, int rem, int* lv, F& visit,
+                       std::vector<uint8_t>* bounds, decltype(bound_at)& bnd) {
+            int b = bnd(k, rem);
+            if (bounds) bounds->push_back(static_cast<uint8_t>(b));
+            for (int l = 0; l <= b; ++l) {
+                lv[k] = l;
+                if (k == 0) visit(lv);
+                else go(g, k - 1, rem - l, lv, visit, bounds, bnd);
+            }
+            lv[k] = 0;
+        }
+    };
+    (void)d;
+    Rec::go(g, g.d - 1, g.n - 1, lv, visit, bounds, bound_at);
+}
+
+}  // namespace
+
+int build_launch_tables(Grid& g, int hier_tile_values) {
+    const int d = g.d;
+    // evaluation walk
+    g.dfs_base.clear();
+    g.dfs_base.reserve(g.S);
+    g.dfs_bound.clear();
+    int64_t visited = 0;
+    bool ok = true;
+    dfs_walk(g, [&](const int* lv) {
+        int64_t id = subspace_id(g, lv);
+        if (id < 0) { ok = false; return; }
+        g.dfs_base.push_back(g.sub_base[id]);
+        ++visited;
+    }, g.explicit_set ? &g.dfs_bound : nullptr);
+    if (!ok || visited != g.S) {
+        set_error("build_launch_tables: depth-first walk does not cover the level set");
+        return kInvalid;
+    }
+    // hierarchization: pole families per dimension
+    g.hier.assign(d, HierTables());
+    int lv[kMaxDims];
+    for (int k = 0; k < d; ++k) {
+        HierTables& h = g.hier[k];
+        int32_t tile_first = 0, tile_fams = 0, tile_vals = 0;
+        auto close_tile = [&]() {
+            if (tile_fams == 0) return;
+            h.tiles.push_back(tile_first);
+            h.tiles.push_back(tile_fams);
+            h.tiles.push_back(tile_vals);
+            h.max_families_per_tile = std::max(h.max_families_per_tile, int(tile_fams));
+            tile_first += tile_fams;
+            tile_fams = 0;
+            tile_vals = 0;
+        };
+        for (int64_t id = 0; id < g.S; ++id) {
+            const uint8_t* l = &g.sub_levels[id * d];
+            if (l[k] != 0) continue;               // families are rooted at l_k = 0
+            int c = 0, B = 0;
+            for (int j = 0; j < d; ++j) {
+                lv[j] = l[j];
+                c += l[j];
+                if (j > k) B += l[j];
+            }
+            std::vector<int32_t> bases;
+            for (int m = 0;; ++m) {
+                lv[k] = m;
+                int64_t sid = subspace_id(g, lv);
+                if (sid < 0) break;
+                bases.push_back(g.sub_base[sid]);
+            }
+            int m = int(bases.size()) - 1;
+            if (m == 0) continue;                   // l_k = 0 points have no parents in k
+            int64_t vals = (int64_t(1) << c) * ((int64_t(2) << m) - 1);
+            if (vals > (int64_t(1) << 24)) {
+                set_error("build_launch_tables: pole family too large");
+                return kUnsupported;
+            }
+            h.max_family_values = std::max<int>(h.max_family_values, int(vals));
+            if (tile_fams > 0 && tile_vals + vals > hier_tile_values) close_tile();
+            h.fams.push_back(tile_vals);
+            h.fams.push_back(m | (c << 8) | (B << 16));
+            h.fams.push_back(int32_t(h.blocks.size()));
+            h.blocks.insert(h.blocks.end(), bases.begin(), bases.end());
+            tile_vals += int32_t(vals);
+            ++tile_fams;
+        }
+        close_tile();
+    }
+    return kOk;
+}
+
+}  // namespace sg

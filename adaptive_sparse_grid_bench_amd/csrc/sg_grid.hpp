@@ -1,0 +1,87 @@
+// Host-side description of one compact sparse grid: shape tables, the subspace list in
+// compact order, the offset <-> (levels, indices) bijection, and the launch tables the
+// HIP kernels walk.
+//
+// Compact order (adaptive_sparse_grid_bench.c, "asg.c", 187-243): subspaces sorted by
+// level sum s ascending, then by rank within s (asg.c:266-279: l_{d-1} most significant,
+// ascending); a subspace l holds 2^s points stored row-major over its per-dimension
+// indices with dimension 0 most significant.  Levels are 0-based; point (l, i) sits at
+// x_k = (2 i_k + 1) / 2^(l_k + 1) (asg.c:150).
+//
+// Two kinds of level sets are supported:
+//   * restricted-regular: { l : |l|_1 <= n-1, l_k <= limits[k] } (asg.c:82-83, 108-129);
+//     limits = n-1 everywhere is the regular grid.  Ranks come from the dp matrix.
+//   * explicit: any downward-closed set of level vectors given by the caller (the
+//     refined-subspace form for adaptive grids).  Ranks are positions in the sorted list,
+//     sorted by the same key, so a restricted-regular set given explicitly lands in the
+//     identical order.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace sg {
+
+constexpr int kMaxDims = 20;        // asg.cu:17 (MAX_NUM_DIMS)
+constexpr int kMaxLevel = 15;       // per-dimension level cap of the kernels' tables
+constexpr int kMaxLevelSum = 26;    // keeps 2^s * sizeof(double) inside 32-bit offsets
+
+struct HierTables {
+    // One entry per tile / family / block of the pole-family decomposition of one
+    // dimension.  A family is the set of subspaces that differ only in l_k; its blocks
+    // are those subspaces, l_k = 0..m.
+    std::vector<int32_t> tiles;     // [T][3]: first family, family count, value count
+    std::vector<int32_t> fams;      // [F][3]: first value in the tile, m|c<<8|B<<16, first block
+    std::vector<int32_t> blocks;    // base offset of each block
+    int max_family_values = 0;
+    int max_families_per_tile = 0;
+};
+
+struct Grid {
+    int d = 0;
+    int n = 0;                      // number of level sums: 0..n-1
+    bool explicit_set = false;
+    int limits[kMaxDims] = {};
+    int max_level[kMaxDims] = {};   // largest l_k that occurs
+    std::vector<int64_t> dp;        // [d][n] (restricted-regular only)
+    std::vector<int64_t> lidx;      // [n+1]: first offset of level sum s; lidx[n] = N
+    std::vector<int64_t> cnt;       // [n+1]: subspaces before level sum s; cnt[n] = S
+    int64_t N = 0;
+    int64_t S = 0;
+    std::vector<uint8_t> sub_levels;  // [S][d] in compact order
+    std::vector<int32_t> sub_base;    // [S] first offset of each subspace
+    std::unordered_map<uint64_t, int64_t> index_of;  // explicit sets: key(l) -> subspace id
+
+    // evaluation walk: subspaces in depth-first order (dims d-1 outermost ... 0 innermost)
+    std::vector<int32_t> dfs_base;    // [S] base offset per leaf, in walk order
+    std::vector<uint8_t> dfs_bound;   // explicit sets: per walk node, the largest l_k
+    std::vector<HierTables> hier;     // [d]
+};
+
+// status codes shared with the C ABI
+enum Status : int {
+    kOk = 0,
+    kInvalid = -1,
+    kNoMem = -2,
+    kHip = -3,
+    kRange = -4,
+    kUnsupported = -5,
+};
+
+const char* status_name(int st);
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+int build_regular(Grid& g, int d, int n, const int* limits);
+int build_explicit(Grid& g, int d, const uint8_t* levels, int64_t count);
+int build_launch_tables(Grid& g, int hier_tile_values);
+
+int64_t subspace_id(const Grid& g, const int* levels);   // -1 if absent
+int64_t l2idx(const Grid& g, const int* levels);        // rank within its level sum
+int idx2l(const Grid& g, int64_t rank, int sum, int* levels);
+int64_t gp2idx(const Grid& g, const int* levels, const int* indices);  // -1 if absent
+int idx2gp(const Grid& g, int64_t offset, int* levels, int* indices);
+
+}  // namespace sg

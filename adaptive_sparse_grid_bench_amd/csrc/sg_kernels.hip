@@ -1,0 +1,778 @@
+// HIP kernels of the compact sparse grid hot path, written for gfx950 (CDNA4, wave64).
+//
+//   k_idx2gp / k_gp2idx   the offset <-> (levels, indices) bijection, asg.c:187-243
+//   k_grid_coords         grid-point coordinates (2i+1)/2^(l+1) per offset, asg.c:143-152
+//   k_hierarchize_dim     one dimension's sweep v -= (left + right)/2 over pole families
+//                         held in LDS, asg.c:557-571 semantics
+//   k_eval_dfs            interpolant at one point per lane, depth-first walk over the
+//                         level set with incremental hat products, asg.c:1237-1274 values
+//   k_eval_ref            the same interpolant in the reference's exact operation order
+//                         (bit-identical to evaluate0)
+//   k_uniform             seeded query points k * 2^-24 (splitmix64 counter stream)
+//
+// ("asg.c" = adaptive_sparse_grid_bench.c of the reference benchmark.)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "sg_kernels.hpp"
+
+namespace sg {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+#define SG_HIP_CHECK(expr)                                                        \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+            return kHip;                                                          \
+        }                                                                         \
+    } while (0)
+
+int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return kHip;
+    }
+    return kOk;
+}
+
+// --------------------------------------------------------------------- bijection --
+
+struct BijParams {
+    int d;
+    int n;
+    int explicit_set;
+    int64_t N;
+    const int32_t* dp;          // [d][n]
+    const int32_t* lidx;        // [n+1]
+    const int32_t* cnt;         // [n+1]
+    const uint8_t* sub_levels;  // [S][d]
+    const int32_t* sub_base;    // [S]
+    int limits[kMaxDims];
+};
+
+constexpr int kMaxN = kMaxLevelSum + 1;
+
+__device__ __forceinline__ void decode_offset(const BijParams& p, const int32_t* sdp,
+                                              const int32_t* slidx, const int32_t* scnt,
+                                              int64_t off, int32_t* lv, int32_t* iv) {
+    int s = 0;
+    while (s + 1 < p.n && off >= slidx[s + 1]) ++s;
+    int64_t r = off - slidx[s];
+    int64_t rank = r >> s;
+    uint32_t bits = static_cast<uint32_t>(r & ((int64_t(1) << s) - 1));
+    if (p.explicit_set) {
+        const uint8_t* l = p.sub_levels + (int64_t(scnt[s]) + rank) * p.d;
+        for (int i = 0; i < p.d; ++i) lv[i] = l[i];
+    } else {
+        // greedy descent through dp, asg.c:221-236
+        int sum = s;
+        for (int i = p.d - 1; i >= 1; --i) {
+            int l = 0;
+            while (l <= sum && rank >= sdp[(i - 1) * p.n + (sum - l)]) {
+                rank -= sdp[(i - 1) * p.n + (sum - l)];
+                ++l;
+            }
+            lv[i] = l;
+            sum -= l;
+        }
+        lv[0] = sum;
+    }
+    for (int i = p.d - 1; i >= 0; --i) {
+        iv[i] = static_cast<int32_t>(bits & ((1u << lv[i]) - 1u));
+        bits >>= lv[i];
+    }
+}
+
+__device__ __forceinline__ void load_bij_tables(const BijParams& p, int32_t* sdp,
+                                                int32_t* slidx, int32_t* scnt) {
+    if (!p.explicit_set)
+        for (int i = threadIdx.x; i < p.d * p.n; i += blockDim.x) sdp[i] = p.dp[i];
+    for (int i = threadIdx.x; i <= p.n; i += blockDim.x) {
+        slidx[i] = p.lidx[i];
+        scnt[i] = p.cnt[i];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kThreads) void k_idx2gp(BijParams p, const int32_t* index,
+                                                     int64_t count, int32_t* levels,
+                                                     int32_t* indices) {
+    __shared__ int32_t sdp[kMaxDims * kMaxN];
+    __shared__ int32_t slidx[kMaxN + 1];
+    __shared__ int32_t scnt[kMaxN + 1];
+    load_bij_tables(p, sdp, slidx, scnt);
+    int32_t lv[kMaxDims], iv[kMaxDims];
+    for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < count;
+         o += int64_t(gridDim.x) * blockDim.x) {
+        int64_t off = index ? int64_t(index[o]) : o;
+        int32_t* lo = levels + o * p.d;
+        int32_t* io = indices + o * p.d;
+        if (off < 0 || off >= p.N) {
+            for (int i = 0; i < p.d; ++i) { lo[i] = -1; io[i] = -1; }
+            continue;
+        }
+        decode_offset(p, sdp, slidx, scnt, off, lv, iv);
+        for (int i = 0; i < p.d; ++i) { lo[i] = lv[i]; io[i] = iv[i]; }
+    }
+}
+
+// compact-order comparison of a level vector against subspace q (explicit sets)
+__device__ __forceinline__ int cmp_levels(const int32_t* lv, const uint8_t* l, int d) {
+    for (int j = d - 1; j >= 0; --j) {
+        if (lv[j] != l[j]) return lv[j] < l[j] ? -1 : 1;
+    }
+    return 0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_gp2idx(BijParams p, const int32_t* levels,
+                                                     const int32_t* indices, int64_t count,
+                                                     int32_t* index) {
+    __shared__ int32_t sdp[kMaxDims * kMaxN];
+    __shared__ int32_t slidx[kMaxN + 1];
+    __shared__ int32_t scnt[kMaxN + 1];
+    load_bij_tables(p, sdp, slidx, scnt);
+    int32_t lv[kMaxDims];
+    for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < count;
+         o += int64_t(gridDim.x) * blockDim.x) {
+        const int32_t* li = levels + o * p.d;
+        const int32_t* ii = indices + o * p.d;
+        int sum = 0;
+        bool ok = true;
+        uint32_t in_block = 0;
+        for (int j = 0; j < p.d; ++j) {
+            int l = li[j], i = ii[j];
+            lv[j] = l;
+            ok = ok && l >= 0 && l <= p.limits[j] && i >= 0 && i < (1 << l);
+            sum += l;
+            in_block = (in_block << (ok ? l : 0)) + uint32_t(ok ? i : 0);
+        }
+        ok = ok && sum < p.n;
+        if (!ok) { index[o] = -1; continue; }
+        int64_t rank;
+        if (p.explicit_set) {
+            int64_t lo = scnt[sum], hi = scnt[sum + 1];   // binary search in [lo, hi)
+            while (lo < hi) {
+                int64_t mid = (lo + hi) >> 1;
+                if (cmp_levels(lv, p.sub_levels + mid * p.d, p.d) > 0) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo >= scnt[sum + 1] || cmp_levels(lv, p.sub_levels + lo * p.d, p.d) != 0) {
+                index[o] = -1;
+                continue;
+            }
+            rank = lo - scnt[sum];
+        } else {
+            // asg.c:195-201
+            rank = 0;
+            int part = lv[0];
+            for (int i = 1; i < p.d; ++i) {
+                part += lv[i];
+                for (int j = 0; j < lv[i]; ++j) rank += sdp[(i - 1) * p.n + (part - j)];
+            }
+        }
+        index[o] = static_cast<int32_t>(slidx[sum] + (rank << sum) + in_block);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_grid_coords(BijParams p, double* coords) {
+    __shared__ int32_t sdp[kMaxDims * kMaxN];
+    __shared__ int32_t slidx[kMaxN + 1];
+    __shared__ int32_t scnt[kMaxN + 1];
+    load_bij_tables(p, sdp, slidx, scnt);
+    int32_t lv[kMaxDims], iv[kMaxDims];
+    for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < p.N;
+         o += int64_t(gridDim.x) * blockDim.x) {
+        decode_offset(p, sdp, slidx, scnt, o, lv, iv);
+        for (int i = 0; i < p.d; ++i)
+            coords[o * p.d + i] = ldexp(2.0 * iv[i] + 1.0, -(lv[i] + 1));
+    }
+}
+
+BijParams make_bij(const Grid& g, const DeviceTables& t) {
+    BijParams p{};
+    p.d = g.d;
+    p.n = g.n;
+    p.explicit_set = g.explicit_set ? 1 : 0;
+    p.N = g.N;
+    p.dp = t.dp;
+    p.lidx = t.lidx;
+    p.cnt = t.cnt;
+    p.sub_levels = t.sub_levels;
+    p.sub_base = t.sub_base;
+    for (int j = 0; j < g.d; ++j) p.limits[j] = g.limits[j];
+    return p;
+}
+
+int grid_for(int64_t work) {
+    int64_t b = (work + kThreads - 1) / kThreads;
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(b, 65536)));
+}
+
+// ----------------------------------------------------------------- hierarchize --
+
+// One workgroup per tile of pole families of dimension k.  A family is the set of
+// subspaces that share every level but l_k; its blocks (l_k = 0..m) are loaded into
+// LDS one after another, so block j of a family with c = sum of the other levels
+// starts at 2^c (2^j - 1) and holds 2^(c+j) values.  Inside block j the in-block index
+// splits as hi | i_k | lo with widths (c - B, j, B), B = sum of the levels of dims > k
+// (asg.c:893-913).  Every point's parents in dimension k (asg.c:828-854) lie in the same
+// family, so the sweep reads only LDS and writes each value once, in place.
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_hierarchize_dim(T* __restrict__ v,
+                                                              const int32_t* __restrict__ tiles,
+                                                              const int32_t* __restrict__ fams,
+                                                              const int32_t* __restrict__ blocks,
+                                                              int lds_values) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* buf = reinterpret_cast<T*>(smem);
+    int32_t* fstart = reinterpret_cast<int32_t*>(smem + size_t(lds_values) * sizeof(T));
+    const int f0 = tiles[3 * blockIdx.x];
+    const int nf = tiles[3 * blockIdx.x + 1];
+    const int nv = tiles[3 * blockIdx.x + 2];
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) fstart[i] = fams[3 * (f0 + i)];
+    __syncthreads();
+
+    auto locate = [&](int val, int& f, int& j, int& r, int& c, int& B, int& bfirst) {
+        int lo = 0, hi = nf - 1;           // last family whose start is <= val
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (fstart[mid] <= val) lo = mid;
+            else hi = mid - 1;
+        }
+        f = lo;
+        const int meta = fams[3 * (f0 + f) + 1];
+        bfirst = fams[3 * (f0 + f) + 2];
+        c = (meta >> 8) & 255;
+        B = meta >> 16;
+        const int local = val - fstart[f];
+        j = 31 - __clz((local >> c) + 1);
+        r = local - (((1 << j) - 1) << c);
+    };
+
+    for (int val = threadIdx.x; val < nv; val += blockDim.x) {
+        int f, j, r, c, B, bfirst;
+        locate(val, f, j, r, c, B, bfirst);
+        buf[val] = v[blocks[bfirst + j] + r];
+    }
+    __syncthreads();
+    for (int val = threadIdx.x; val < nv; val += blockDim.x) {
+        int f, j, r, c, B, bfirst;
+        locate(val, f, j, r, c, B, bfirst);
+        if (j == 0) continue;                  // l_k = 0: no parents in dimension k
+        const int fbase = fstart[f];
+        const int lo = r & ((1 << B) - 1);
+        const int ik = (r >> B) & ((1 << j) - 1);
+        const int hi = r >> (B + j);
+        T left = T(0), right = T(0);
+        if (ik != 0) {                          // left parent, asg.c:828-840
+            const int tz = __builtin_ctz(ik);
+            const int pl = j - 1 - tz;
+            const int pi = ik >> (tz + 1);
+            left = buf[fbase + (((1 << pl) - 1) << c) + (hi << (B + pl)) + (pi << B) + lo];
+        }
+        if (ik != (1 << j) - 1) {               // right parent, asg.c:842-854
+            const int tz = __builtin_ctz(ik + 1);
+            const int pl = j - 1 - tz;
+            const int pi = (ik + 1) >> (tz + 1);
+            right = buf[fbase + (((1 << pl) - 1) << c) + (hi << (B + pl)) + (pi << B) + lo];
+        }
+        v[blocks[bfirst + j] + r] = buf[val] - (left + right) * T(0.5);
+    }
+}
+
+template <class T>
+int hierarchize_impl(const Grid& g, const DeviceTables& t, T* v, hipStream_t s) {
+    for (int k = 0; k < g.d; ++k) {
+        const DeviceTables::Dim& dm = t.dims[k];
+        if (dm.num_tiles == 0) continue;
+        size_t lds = size_t(dm.tile_values) * sizeof(T) + size_t(dm.max_families) * 4;
+        if (lds > 64 * 1024) {
+            set_error("hierarchize: pole-family tile exceeds 64 KiB of LDS");
+            return kUnsupported;
+        }
+        hipLaunchKernelGGL(k_hierarchize_dim<T>, dim3(dm.num_tiles), dim3(kThreads), lds, s, v,
+                           dm.tiles, dm.fams, dm.blocks, dm.tile_values);
+        int st = check_launch("k_hierarchize_dim");
+        if (st != kOk) return st;
+    }
+    return kOk;
+}
+
+// -------------------------------------------------------------------- evaluate --
+
+// Hat values and in-block byte offsets of one coordinate at levels 0..LT-1.
+// With m_l = x 2^(l+1) - (2 t_l + 1), t_l = floor(x 2^l):  hat_l = 1 - |m_l|,
+// t_{l+1} = 2 t_l + [m_l >= 0],  m_{l+1} = 2 m_l - (m_l >= 0 ? 1 : -1).  For x on the
+// 2^-24 lattice every step is exact, so hat_l equals asg.c:1257-1262's value.
+template <int LT, class T>
+__device__ __forceinline__ void level_table(T x, T (&ph)[LT], uint32_t (&ub)[LT]) {
+    T m = T(2) * x - T(1);
+    uint32_t t = 0;
+#pragma unroll
+    for (int l = 0; l < LT; ++l) {
+        ph[l] = T(1) - fabs(m);
+        ub[l] = t * uint32_t(sizeof(T));
+        const bool ge = m >= T(0);
+        t = 2u * t + (ge ? 1u : 0u);
+        m = T(2) * m + (ge ? T(-1) : T(1));
+    }
+}
+
+struct AccF64 {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double w, double a) { s = fma(w, a, s); }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
+// Kahan-compensated FP32 accumulation across subspaces
+struct AccKahan {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float w, float a) {
+        const float y = fmaf(w, a, -c);
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s; }
+};
+
+template <class T>
+struct EvalParams {
+    const T* alpha;
+    const T* coords;            // [P][D] row-major
+    T* out;
+    int64_t P;
+    const int32_t* leaf_base;   // subspace base offsets in walk order
+    int n;                      // level sums 0..n-1
+    int limits[kMaxDims];
+};
+
+template <class T, class Acc>
+struct WalkState {
+    const unsigned char* alpha;
+    const int32_t* leaf_base;
+    int q;
+    Acc acc;
+    __device__ __forceinline__ void leaf(T w, uint32_t voff) {
+        const unsigned char* blk = alpha + size_t(leaf_base[q]) * sizeof(T);
+        ++q;
+        acc.add(w, *reinterpret_cast<const T*>(blk + voff));
+    }
+};
+
+// Per-lane state.  With LT <= 8 the hats and offsets of dims 0 and 1 at every level sit
+// in register tables (the two innermost loops are unrolled over them); with LT = 16 the
+// tables would not fit the unroller's budget and every dimension is walked incrementally.
+template <int D, int LT, class T>
+struct Lane {
+    static constexpr int kTab = LT <= 8 ? LT : 1;
+    T m0[D];                        // 2 x_k - 1 (m at level 0)
+    T ph0[kTab], ph1[kTab];         // hats of dims 0 and 1 at every level
+    uint32_t ub0[kTab], ub1[kTab];  // t_l * sizeof(T) of dims 0 and 1
+};
+
+// Depth-first walk: dimension K's level runs 0..min(rem, limit_K); dimension d-1 is
+// outermost, so for a fixed level sum the leaves come in rank order.  pp is the
+// product of the hats of dims > K, off the in-block byte offset they contribute,
+// hsum the sum of their levels (the shift of dimension K's index bits).
+template <int K, int D, int LT, class T, class Acc>
+struct Walker {
+    static __device__ __forceinline__ void run(const Lane<D, LT, T>& ln, const int* lim,
+                                               T pp, uint32_t off, int hsum, int rem,
+                                               WalkState<T, Acc>& st) {
+        const int b = min(rem, lim[K]);
+        T m = ln.m0[K];
+        uint32_t t = 0;
+        for (int l = 0; l <= b; ++l) {
+            const T phi = T(1) - fabs(m);
+            Walker<K - 1, D, LT, T, Acc>::run(ln, lim, pp * phi,
+                                              off + ((t * uint32_t(sizeof(T))) << hsum),
+                                              hsum + l, rem - l, st);
+            const bool ge = m >= T(0);
+            t = 2u * t + (ge ? 1u : 0u);
+            m = T(2) * m + (ge ? T(-1) : T(1));
+        }
+    }
+};
+
+// Dims 1 and 0 from the register tables, both loops unrolled (LT <= 8 keeps the
+// LT x LT bodies within the unroller's budget; LT = 16 walks dim 1 like the others).
+template <int D, int LT, class T, class Acc>
+struct Walker<1, D, LT, T, Acc> {
+    static __device__ __forceinline__ void run(const Lane<D, LT, T>& ln, const int* lim,
+                                               T pp, uint32_t off, int hsum, int rem,
+                                               WalkState<T, Acc>& st) {
+        if constexpr (LT > 8) {
+            const int b = min(rem, lim[1]);
+            T m = ln.m0[1];
+            uint32_t t = 0;
+            for (int l = 0; l <= b; ++l) {
+                const T phi = T(1) - fabs(m);
+                Walker<0, D, LT, T, Acc>::run(ln, lim, pp * phi,
+                                              off + ((t * uint32_t(sizeof(T))) << hsum),
+                                              hsum + l, rem - l, st);
+                const bool ge = m >= T(0);
+                t = 2u * t + (ge ? 1u : 0u);
+                m = T(2) * m + (ge ? T(-1) : T(1));
+            }
+        } else {
+            const int b1 = min(rem, lim[1]);
+#pragma unroll
+            for (int l1 = 0; l1 < LT; ++l1) {
+                if (l1 > b1) break;
+                const T pp1 = pp * ln.ph1[l1];
+                const uint32_t off1 = (ln.ub1[l1] << hsum) + off;
+                const int h1 = hsum + l1;
+                const int b0 = min(rem - l1, lim[0]);
+#pragma unroll
+                for (int l0 = 0; l0 < LT; ++l0) {
+                    if (l0 > b0) break;
+                    st.leaf(pp1 * ln.ph0[l0], (ln.ub0[l0] << h1) + off1);
+                }
+            }
+        }
+    }
+};
+
+template <int D, int LT, class T, class Acc>
+struct Walker<0, D, LT, T, Acc> {
+    static __device__ __forceinline__ void run(const Lane<D, LT, T>& ln, const int* lim,
+                                               T pp, uint32_t off, int hsum, int rem,
+                                               WalkState<T, Acc>& st) {
+        if constexpr (LT > 8) {                 // no tables: incremental hats as above
+            const int b = min(rem, lim[0]);
+            T m = ln.m0[0];
+            uint32_t t = 0;
+            for (int l = 0; l <= b; ++l) {
+                st.leaf(pp * (T(1) - fabs(m)), ((t * uint32_t(sizeof(T))) << hsum) + off);
+                const bool ge = m >= T(0);
+                t = 2u * t + (ge ? 1u : 0u);
+                m = T(2) * m + (ge ? T(-1) : T(1));
+            }
+        } else {
+            const int b0 = min(rem, lim[0]);
+#pragma unroll
+            for (int l0 = 0; l0 < LT; ++l0) {
+                if (l0 > b0) break;
+                st.leaf(pp * ln.ph0[l0], (ln.ub0[l0] << hsum) + off);
+            }
+        }
+    }
+};
+
+template <int D, int LT, class T, class Acc>
+__global__ __launch_bounds__(kThreads) void k_eval_dfs(EvalParams<T> prm) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool live = p < prm.P;
+    const int64_t pc = live ? p : 0;     // idle lanes walk point 0 and store nothing
+    Lane<D, LT, T> ln;
+    const T* x = prm.coords + pc * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const T xk = x[k];
+        if constexpr (LT <= 8) {
+            if (k == 0) level_table<LT, T>(xk, ln.ph0, ln.ub0);
+            else if (k == 1) level_table<LT, T>(xk, ln.ph1, ln.ub1);
+        }
+        ln.m0[k] = T(2) * xk - T(1);
+    }
+    WalkState<T, Acc> st{reinterpret_cast<const unsigned char*>(prm.alpha), prm.leaf_base, 0,
+                         Acc()};
+    Walker<D - 1, D, LT, T, Acc>::run(ln, prm.limits, T(1), 0u, 0, prm.n - 1, st);
+    if (live) prm.out[p] = st.acc.value();
+}
+
+template <int D, int LT, class T, class Acc>
+int launch_dfs(const EvalParams<T>& prm, int block, hipStream_t s) {
+    const int64_t blocks = (prm.P + block - 1) / block;
+    hipLaunchKernelGGL((k_eval_dfs<D, LT, T, Acc>), dim3(unsigned(blocks)), dim3(block), 0, s,
+                       prm);
+    return check_launch("k_eval_dfs");
+}
+
+template <int LT, class T, class Acc>
+int dispatch_dfs_d(int d, const EvalParams<T>& prm, int block, hipStream_t s) {
+    switch (d) {
+#define SG_CASE(DD) \
+    case DD: return launch_dfs<DD, LT, T, Acc>(prm, block, s);
+        SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6) SG_CASE(7)
+        SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12) SG_CASE(13) SG_CASE(14)
+        SG_CASE(15) SG_CASE(16) SG_CASE(17) SG_CASE(18) SG_CASE(19) SG_CASE(20)
+#undef SG_CASE
+        default:
+            set_error("evaluate: unsupported number of dimensions");
+            return kUnsupported;
+    }
+}
+
+template <class T, class Acc>
+int evaluate_impl(const Grid& g, const DeviceTables& t, const T* alpha, const T* coords,
+                  int64_t P, T* out, const EvalTuning& tu, hipStream_t s) {
+    if (P == 0) return kOk;
+    if (g.explicit_set) {
+        set_error("evaluate: explicit level sets use the refined-subspace entry point");
+        return kUnsupported;
+    }
+    if (tu.block != 64 && tu.block != 128 && tu.block != 256) {
+        set_error("evaluate: block must be 64, 128 or 256");
+        return kInvalid;
+    }
+    if ((P + tu.block - 1) / tu.block > int64_t(0x7fffffff)) {
+        set_error("evaluate: too many points for one launch");
+        return kRange;
+    }
+    EvalParams<T> prm{};
+    prm.alpha = alpha;
+    prm.coords = coords;
+    prm.out = out;
+    prm.P = P;
+    prm.leaf_base = t.dfs_base;
+    prm.n = g.n;
+    int maxl = 0;
+    for (int j = 0; j < g.d; ++j) {
+        prm.limits[j] = g.limits[j];
+        maxl = std::max(maxl, g.limits[j]);
+    }
+    // dims 0 and 1 keep per-level tables of LT entries in registers
+    const int need = std::max(g.limits[0], g.d > 1 ? g.limits[1] : 0) + 1;
+    if (need <= 8) return dispatch_dfs_d<8, T, Acc>(g.d, prm, tu.block, s);
+    if (need <= 16) return dispatch_dfs_d<16, T, Acc>(g.d, prm, tu.block, s);
+    set_error("evaluate: level above 15 in dimension 0 or 1");
+    return kUnsupported;
+}
+
+// Reference operation order (asg.c:1250-1268): for each subspace in compact order the hat
+// product over k = 0..d-1 starting from 1, times the surplus, added to the running sum.
+// Division by div = 2^-l is exact, so it is written as ldexp.  No contraction.
+template <int D, class T>
+__global__ __launch_bounds__(kThreads) void k_eval_ref(const T* __restrict__ alpha,
+                                                       const T* __restrict__ coords,
+                                                       T* __restrict__ out, int64_t P,
+                                                       const uint8_t* __restrict__ sub_levels,
+                                                       const int32_t* __restrict__ sub_base,
+                                                       int64_t S) {
+#pragma clang fp contract(off)
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool live = p < P;
+    const T* xp = coords + (live ? p : 0) * D;
+    T x[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = xp[k];
+    T acc = T(0);
+    for (int64_t j = 0; j < S; ++j) {
+        const uint8_t* lv = sub_levels + j * D;
+        T prod = T(1);
+        int32_t index2 = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const int l = lv[k];
+            const T div = ldexp(T(1), -l);
+            const int t = static_cast<int>(ldexp(x[k], l));
+            index2 = index2 * (1 << l) + t;
+            const T left = T(t) * div;
+            const T m = ldexp(T(2) * (x[k] - left) - div, l);
+            prod *= T(1) + m * (T(1) - T((m >= T(0)) << 1));
+        }
+        prod *= alpha[sub_base[j] + index2];
+        acc += prod;
+    }
+    if (live) out[p] = acc;
+}
+
+template <class T>
+int evaluate_ref_impl(const Grid& g, const DeviceTables& t, const T* alpha, const T* coords,
+                      int64_t P, T* out, hipStream_t s) {
+    if (P == 0) return kOk;
+    const int64_t blocks = (P + kThreads - 1) / kThreads;
+    if (blocks > int64_t(0x7fffffff)) {
+        set_error("evaluate_ref: too many points for one launch");
+        return kRange;
+    }
+    switch (g.d) {
+#define SG_CASE(DD)                                                                        \
+    case DD:                                                                               \
+        hipLaunchKernelGGL((k_eval_ref<DD, T>), dim3(unsigned(blocks)), dim3(kThreads), 0, \
+                           s, alpha, coords, out, P, t.sub_levels, t.sub_base, g.S);       \
+        return check_launch("k_eval_ref");
+        SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6) SG_CASE(7)
+        SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12) SG_CASE(13) SG_CASE(14)
+        SG_CASE(15) SG_CASE(16) SG_CASE(17) SG_CASE(18) SG_CASE(19) SG_CASE(20)
+#undef SG_CASE
+        default:
+            set_error("evaluate_ref: unsupported number of dimensions");
+            return kUnsupported;
+    }
+}
+
+// ------------------------------------------------------------------ utilities --
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_uniform(int64_t count, uint64_t seed, T* out) {
+    for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < count;
+         j += int64_t(gridDim.x) * blockDim.x) {
+        const uint64_t k = mix64((seed << 32) + uint64_t(j)) >> 40;
+        out[j] = T(ldexp(double(k), -24));
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_f64_to_f32(const double* in, float* out,
+                                                         int64_t count) {
+    for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < count;
+         j += int64_t(gridDim.x) * blockDim.x)
+        out[j] = float(in[j]);
+}
+
+template <class T>
+void free_dev(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <class T>
+int upload(T*& dst, const T* src, size_t count) {
+    if (count == 0) return kOk;
+    hipError_t e = hipMalloc(&dst, count * sizeof(T));
+    if (e != hipSuccess) {
+        dst = nullptr;
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? kNoMem : kHip;
+    }
+    SG_HIP_CHECK(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return kOk;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- launchers --
+
+int upload_tables(const Grid& g, DeviceTables& t) {
+    SG_HIP_CHECK(hipGetDevice(&t.device));
+    std::vector<int32_t> dp32(g.dp.size()), lidx32(g.lidx.size()), cnt32(g.cnt.size());
+    for (size_t i = 0; i < g.dp.size(); ++i)
+        dp32[i] = static_cast<int32_t>(std::min<int64_t>(g.dp[i], 0x7fffffff));
+    for (size_t i = 0; i < g.lidx.size(); ++i) lidx32[i] = static_cast<int32_t>(g.lidx[i]);
+    for (size_t i = 0; i < g.cnt.size(); ++i) cnt32[i] = static_cast<int32_t>(g.cnt[i]);
+    int st;
+    if ((st = upload(t.dp, dp32.data(), dp32.size())) != kOk) return st;
+    if ((st = upload(t.lidx, lidx32.data(), lidx32.size())) != kOk) return st;
+    if ((st = upload(t.cnt, cnt32.data(), cnt32.size())) != kOk) return st;
+    if ((st = upload(t.sub_levels, g.sub_levels.data(), g.sub_levels.size())) != kOk) return st;
+    if ((st = upload(t.sub_base, g.sub_base.data(), g.sub_base.size())) != kOk) return st;
+    if ((st = upload(t.dfs_base, g.dfs_base.data(), g.dfs_base.size())) != kOk) return st;
+    if ((st = upload(t.dfs_bound, g.dfs_bound.data(), g.dfs_bound.size())) != kOk) return st;
+    for (int k = 0; k < g.d; ++k) {
+        const HierTables& h = g.hier[k];
+        DeviceTables::Dim& dm = t.dims[k];
+        dm.num_tiles = int(h.tiles.size() / 3);
+        dm.max_families = h.max_families_per_tile;
+        dm.tile_values = 0;
+        for (int i = 0; i < dm.num_tiles; ++i) dm.tile_values = std::max(dm.tile_values, h.tiles[3 * i + 2]);
+        if ((st = upload(dm.tiles, h.tiles.data(), h.tiles.size())) != kOk) return st;
+        if ((st = upload(dm.fams, h.fams.data(), h.fams.size())) != kOk) return st;
+        if ((st = upload(dm.blocks, h.blocks.data(), h.blocks.size())) != kOk) return st;
+    }
+    return kOk;
+}
+
+void free_tables(DeviceTables& t) {
+    int cur = -1;
+    bool switched = false;
+    if (t.device >= 0 && hipGetDevice(&cur) == hipSuccess && cur != t.device)
+        switched = hipSetDevice(t.device) == hipSuccess;
+    free_dev(t.dp);
+    free_dev(t.lidx);
+    free_dev(t.cnt);
+    free_dev(t.sub_levels);
+    free_dev(t.sub_base);
+    free_dev(t.dfs_base);
+    free_dev(t.dfs_bound);
+    for (auto& dm : t.dims) {
+        free_dev(dm.tiles);
+        free_dev(dm.fams);
+        free_dev(dm.blocks);
+        dm.num_tiles = 0;
+    }
+    if (switched) (void)hipSetDevice(cur);
+}
+
+int launch_idx2gp(const Grid& g, const DeviceTables& t, const int32_t* index, int64_t count,
+                  int32_t* levels, int32_t* indices, hipStream_t s) {
+    if (count == 0) return kOk;
+    hipLaunchKernelGGL(k_idx2gp, dim3(grid_for(count)), dim3(kThreads), 0, s, make_bij(g, t),
+                       index, count, levels, indices);
+    return check_launch("k_idx2gp");
+}
+
+int launch_gp2idx(const Grid& g, const DeviceTables& t, const int32_t* levels,
+                  const int32_t* indices, int64_t count, int32_t* index, hipStream_t s) {
+    if (count == 0) return kOk;
+    hipLaunchKernelGGL(k_gp2idx, dim3(grid_for(count)), dim3(kThreads), 0, s, make_bij(g, t),
+                       levels, indices, count, index);
+    return check_launch("k_gp2idx");
+}
+
+int launch_grid_coords_f64(const Grid& g, const DeviceTables& t, double* coords, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_coords, dim3(grid_for(g.N)), dim3(kThreads), 0, s, make_bij(g, t),
+                       coords);
+    return check_launch("k_grid_coords");
+}
+
+int launch_hierarchize_f64(const Grid& g, const DeviceTables& t, double* v, hipStream_t s) {
+    return hierarchize_impl<double>(g, t, v, s);
+}
+int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipStream_t s) {
+    return hierarchize_impl<float>(g, t, v, s);
+}
+
+int launch_evaluate_f64(const Grid& g, const DeviceTables& t, const double* alpha,
+                        const double* coords, int64_t P, double* out, const EvalTuning& tu,
+                        hipStream_t s) {
+    return evaluate_impl<double, AccF64>(g, t, alpha, coords, P, out, tu, s);
+}
+int launch_evaluate_f32c(const Grid& g, const DeviceTables& t, const float* alpha,
+                         const float* coords, int64_t P, float* out, const EvalTuning& tu,
+                         hipStream_t s) {
+    return evaluate_impl<float, AccKahan>(g, t, alpha, coords, P, out, tu, s);
+}
+int launch_evaluate_ref_f64(const Grid& g, const DeviceTables& t, const double* alpha,
+                            const double* coords, int64_t P, double* out, hipStream_t s) {
+    return evaluate_ref_impl<double>(g, t, alpha, coords, P, out, s);
+}
+int launch_evaluate_ref_f32(const Grid& g, const DeviceTables& t, const float* alpha,
+                            const float* coords, int64_t P, float* out, hipStream_t s) {
+    return evaluate_ref_impl<float>(g, t, alpha, coords, P, out, s);
+}
+
+int launch_uniform_points_f64(int64_t count, uint64_t seed, double* out, hipStream_t s) {
+    if (count == 0) return kOk;
+    hipLaunchKernelGGL(k_uniform<double>, dim3(grid_for(count)), dim3(kThreads), 0, s, count,
+                       seed, out);
+    return check_launch("k_uniform");
+}
+int launch_uniform_points_f32(int64_t count, uint64_t seed, float* out, hipStream_t s) {
+    if (count == 0) return kOk;
+    hipLaunchKernelGGL(k_uniform<float>, dim3(grid_for(count)), dim3(kThreads), 0, s, count,
+                       seed, out);
+    return check_launch("k_uniform");
+}
+int launch_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s) {
+    if (count == 0) return kOk;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(count)), dim3(kThreads), 0, s, in, out, count);
+    return check_launch("k_f64_to_f32");
+}
+
+}  // namespace sg

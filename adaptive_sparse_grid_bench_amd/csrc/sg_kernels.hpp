@@ -1,0 +1,66 @@
+// Launchers of the HIP kernels (sg_kernels.hip).  Plain pointers and sizes; every
+// launcher checks its launch shape against the buffers it is given and returns a
+// status from sg::Status.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+#include "sg_grid.hpp"
+
+namespace sg {
+
+// Device copy of the tables one grid's kernels read.  Owned by the grid handle.
+struct DeviceTables {
+    int device = -1;
+    int32_t* dp = nullptr;          // [d][n] int32 (restricted-regular grids)
+    int32_t* lidx = nullptr;        // [n+1]
+    int32_t* cnt = nullptr;         // [n+1]
+    uint8_t* sub_levels = nullptr;  // [S][d]
+    int32_t* sub_base = nullptr;    // [S]
+    int32_t* dfs_base = nullptr;    // [S]
+    uint8_t* dfs_bound = nullptr;   // explicit sets: walk-node bounds
+    struct Dim {
+        int32_t* tiles = nullptr;
+        int32_t* fams = nullptr;
+        int32_t* blocks = nullptr;
+        int num_tiles = 0;
+        int max_families = 0;
+        int tile_values = 0;
+    } dims[kMaxDims];
+    int hier_tile_values = 0;
+};
+
+int upload_tables(const Grid& g, DeviceTables& t);
+void free_tables(DeviceTables& t);
+
+int launch_idx2gp(const Grid& g, const DeviceTables& t, const int32_t* index, int64_t count,
+                  int32_t* levels, int32_t* indices, hipStream_t s);
+int launch_gp2idx(const Grid& g, const DeviceTables& t, const int32_t* levels,
+                  const int32_t* indices, int64_t count, int32_t* index, hipStream_t s);
+int launch_grid_coords_f64(const Grid& g, const DeviceTables& t, double* coords, hipStream_t s);
+
+int launch_hierarchize_f64(const Grid& g, const DeviceTables& t, double* v, hipStream_t s);
+int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipStream_t s);
+
+struct EvalTuning {
+    int block = 256;            // threads per workgroup
+};
+
+int launch_evaluate_f64(const Grid& g, const DeviceTables& t, const double* alpha,
+                        const double* coords, int64_t P, double* out, const EvalTuning& tu,
+                        hipStream_t s);
+int launch_evaluate_f32c(const Grid& g, const DeviceTables& t, const float* alpha,
+                         const float* coords, int64_t P, float* out, const EvalTuning& tu,
+                         hipStream_t s);
+int launch_evaluate_ref_f64(const Grid& g, const DeviceTables& t, const double* alpha,
+                            const double* coords, int64_t P, double* out, hipStream_t s);
+int launch_evaluate_ref_f32(const Grid& g, const DeviceTables& t, const float* alpha,
+                            const float* coords, int64_t P, float* out, hipStream_t s);
+
+int launch_uniform_points_f64(int64_t count, uint64_t seed, double* out, hipStream_t s);
+int launch_uniform_points_f32(int64_t count, uint64_t seed, float* out, hipStream_t s);
+int launch_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s);
+
+}  // namespace sg

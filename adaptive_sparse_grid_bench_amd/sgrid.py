@@ -1,0 +1,248 @@
+"""Python mirror of the reference benchmark's sparse grid operations over libsgrid.so.
+
+``Grid`` carries the reference's operator interface (adaptive_sparse_grid_bench.c,
+"asg.c"): ``gp2idx`` (asg.c:187), ``idx2gp`` (asg.c:210), ``idx2l`` (asg.c:245),
+``l2idx`` (asg.c:266), ``hierarchize`` (asg.c:557, in place) and ``evaluate``
+(asg.c:1237, ``out`` overwritten).  Device buffers are torch tensors on the GPU (torch is
+only the allocator and the stream source here); every computation runs in the HIP
+kernels of libsgrid.so.  The module raises at import if the library is missing: there
+is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsgrid.so")
+
+SG_OK = 0
+SG_E_INVALID = -1
+SG_E_NOMEM = -2
+SG_E_HIP = -3
+SG_E_RANGE = -4
+SG_E_UNSUPPORTED = -5
+
+# every symbol include/sgrid.h declares
+EXPORTS = [
+    "sg_last_error", "sg_status_string", "sg_version",
+    "sg_grid_create", "sg_grid_create_explicit", "sg_grid_destroy", "sg_grid_info",
+    "sg_grid_tables", "sg_grid_subspaces", "sg_grid_set_tuning",
+    "sg_gp2idx", "sg_idx2gp", "sg_idx2l", "sg_l2idx",
+    "sg_idx2gp_dev", "sg_gp2idx_dev", "sg_grid_coords_f64",
+    "sg_hierarchize_f64", "sg_hierarchize_f32",
+    "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
+    "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
+]
+
+
+class SgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(make -C adaptive_sparse_grid_bench_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    vp, ll, i = C.c_void_p, C.c_longlong, C.c_int
+    ip = C.POINTER(C.c_int)
+    llp = C.POINTER(C.c_longlong)
+    sig = {
+        "sg_last_error": (C.c_char_p, []),
+        "sg_status_string": (C.c_char_p, [i]),
+        "sg_version": (i, []),
+        "sg_grid_create": (i, [i, i, ip, C.POINTER(vp)]),
+        "sg_grid_create_explicit": (i, [i, C.POINTER(C.c_ubyte), ll, C.POINTER(vp)]),
+        "sg_grid_destroy": (i, [vp]),
+        "sg_grid_info": (i, [vp, ip, ip, llp, llp]),
+        "sg_grid_tables": (i, [vp, llp, llp]),
+        "sg_grid_subspaces": (i, [vp, C.POINTER(C.c_ubyte), llp]),
+        "sg_grid_set_tuning": (i, [vp, i, i]),
+        "sg_gp2idx": (i, [vp, ip, ip, llp]),
+        "sg_idx2gp": (i, [vp, ll, ip, ip]),
+        "sg_idx2l": (i, [vp, ll, i, ip]),
+        "sg_l2idx": (i, [vp, ip, llp]),
+        "sg_idx2gp_dev": (i, [vp, vp, ll, vp, vp, vp]),
+        "sg_gp2idx_dev": (i, [vp, vp, vp, ll, vp, vp]),
+        "sg_grid_coords_f64": (i, [vp, vp, vp]),
+        "sg_hierarchize_f64": (i, [vp, vp, vp]),
+        "sg_hierarchize_f32": (i, [vp, vp, vp]),
+        "sg_evaluate_f64": (i, [vp, vp, vp, ll, vp, vp]),
+        "sg_evaluate_f32c": (i, [vp, vp, vp, ll, vp, vp]),
+        "sg_evaluate_ref_f64": (i, [vp, vp, vp, ll, vp, vp]),
+        "sg_evaluate_ref_f32": (i, [vp, vp, vp, ll, vp, vp]),
+        "sg_uniform_points_f64": (i, [ll, C.c_ulonglong, vp, vp]),
+        "sg_uniform_points_f32": (i, [ll, C.c_ulonglong, vp, vp]),
+        "sg_convert_f64_to_f32": (i, [vp, vp, ll, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int) -> None:
+    if status != SG_OK:
+        raise SgError(status, lib.sg_last_error().decode())
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _ptr(t) -> int:
+    """Device address of a torch tensor (or an int address, or None)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Grid:
+    """A compact sparse grid resident on the current GPU.
+
+    ``Grid(d, n, limits=None)`` is the restricted-regular grid
+    { l : |l|_1 <= n-1, l_k <= limits[k] } (asg.c:82-83, 101-140);
+    ``Grid.explicit(d, levels)`` takes a downward-closed set of level vectors.
+    """
+
+    def __init__(self, d: int, n: int, limits=None, _handle=None):
+        h = C.c_void_p()
+        if _handle is None:
+            lim = None if limits is None else np.ascontiguousarray(limits, np.int32)
+            check(lib.sg_grid_create(d, n, None if lim is None else _ip(lim), C.byref(h)))
+            self._h = h
+        else:
+            self._h = _handle
+        dd, nn = C.c_int(), C.c_int()
+        NN, SS = C.c_longlong(), C.c_longlong()
+        check(lib.sg_grid_info(self._h, C.byref(dd), C.byref(nn), C.byref(NN), C.byref(SS)))
+        self.d, self.n, self.N, self.S = dd.value, nn.value, NN.value, SS.value
+
+    @classmethod
+    def explicit(cls, d: int, levels) -> "Grid":
+        lv = np.ascontiguousarray(levels, np.uint8).reshape(-1, d)
+        h = C.c_void_p()
+        check(lib.sg_grid_create_explicit(d, lv.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                          lv.shape[0], C.byref(h)))
+        return cls(d, 0, _handle=h)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.sg_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- shape tables --------------------------------------------------------
+    def tables(self):
+        dp = np.zeros(self.d * self.n, np.int64)
+        lidx = np.zeros(self.n + 1, np.int64)
+        check(lib.sg_grid_tables(self._h, dp.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                 lidx.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return dp.reshape(self.d, self.n), lidx
+
+    def subspaces(self):
+        lv = np.zeros((self.S, self.d), np.uint8)
+        base = np.zeros(self.S, np.int64)
+        check(lib.sg_grid_subspaces(self._h, lv.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                    base.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return lv, base
+
+    def set_tuning(self, eval_block: int = 0, hier_tile_values: int = 0) -> None:
+        check(lib.sg_grid_set_tuning(self._h, eval_block, hier_tile_values))
+
+    # ---- host bijection (asg.c:187-279) --------------------------------------
+    def gp2idx(self, levels, indices) -> int:
+        lv = np.ascontiguousarray(levels, np.int32)
+        iv = np.ascontiguousarray(indices, np.int32)
+        out = C.c_longlong()
+        check(lib.sg_gp2idx(self._h, _ip(lv), _ip(iv), C.byref(out)))
+        return out.value
+
+    def idx2gp(self, index: int):
+        lv = np.zeros(self.d, np.int32)
+        iv = np.zeros(self.d, np.int32)
+        check(lib.sg_idx2gp(self._h, index, _ip(lv), _ip(iv)))
+        return lv, iv
+
+    def idx2l(self, rank: int, level_sum: int):
+        lv = np.zeros(self.d, np.int32)
+        check(lib.sg_idx2l(self._h, rank, level_sum, _ip(lv)))
+        return lv
+
+    def l2idx(self, levels) -> int:
+        lv = np.ascontiguousarray(levels, np.int32)
+        out = C.c_longlong()
+        check(lib.sg_l2idx(self._h, _ip(lv), C.byref(out)))
+        return out.value
+
+    # ---- device operations ---------------------------------------------------
+    def idx2gp_dev(self, levels, indices, index=None, count=None, stream=None) -> None:
+        cnt = self.N if count is None else count
+        check(lib.sg_idx2gp_dev(self._h, _ptr(index), cnt, _ptr(levels), _ptr(indices),
+                                _stream(stream)))
+
+    def gp2idx_dev(self, levels, indices, index, count: int, stream=None) -> None:
+        check(lib.sg_gp2idx_dev(self._h, _ptr(levels), _ptr(indices), count, _ptr(index),
+                                _stream(stream)))
+
+    def grid_coords(self, coords, stream=None) -> None:
+        check(lib.sg_grid_coords_f64(self._h, _ptr(coords), _stream(stream)))
+
+    def hierarchize(self, values, stream=None) -> None:
+        """In place; FP64 or FP32 by the tensor's dtype (asg.c:557-571)."""
+        import torch
+        fn = lib.sg_hierarchize_f64 if values.dtype == torch.float64 else lib.sg_hierarchize_f32
+        check(fn(self._h, _ptr(values), _stream(stream)))
+
+    def evaluate(self, alpha, coords, out, stream=None, mode: str = "fast") -> None:
+        """out[p] = interpolant at coords[p] (asg.c:1237-1274).
+
+        mode "fast": FP64 walk (float64 tensors) or FP32 Kahan (float32 tensors);
+        mode "ref": the reference's operation order, bit-identical to evaluate0.
+        """
+        import torch
+        P = coords.shape[0]
+        f64 = alpha.dtype == torch.float64
+        if mode == "fast":
+            fn = lib.sg_evaluate_f64 if f64 else lib.sg_evaluate_f32c
+        elif mode == "ref":
+            fn = lib.sg_evaluate_ref_f64 if f64 else lib.sg_evaluate_ref_f32
+        else:
+            raise ValueError(mode)
+        check(fn(self._h, _ptr(alpha), _ptr(coords), P, _ptr(out), _stream(stream)))
+
+
+def uniform_points(out, seed: int = 42, stream=None) -> None:
+    """Fill a float64/float32 tensor with k * 2^-24 values (splitmix64 stream `seed`)."""
+    import torch
+    fn = lib.sg_uniform_points_f64 if out.dtype == torch.float64 else lib.sg_uniform_points_f32
+    check(fn(out.numel(), seed, _ptr(out), _stream(stream)))
+
+
+def to_f32(src, dst, stream=None) -> None:
+    check(lib.sg_convert_f64_to_f32(_ptr(src), _ptr(dst), src.numel(), _stream(stream)))

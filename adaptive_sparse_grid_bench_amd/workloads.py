@@ -1,0 +1,79 @@
+"""Synthetic inputs of the benchmark workloads: seeded query points and the test
+functions sampled at grid points.
+
+Query points are FP32-representable values k * 2^-24 with k in [0, 2^24), so x = 1 never
+occurs (the reference has no guard for it, asg.c:1259) and FP32 and FP64 paths see the
+same x.  The generator is counter-based splitmix64: value j of stream `seed` is
+mix64(seed * 2^32 + j), so any slice can be regenerated independently (and on the GPU by
+the same formula).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def mix64(z: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser on uint64 (wrap-around arithmetic)."""
+    z = np.asarray(z, np.uint64)
+    with np.errstate(over="ignore"):
+        z = (z + _GAMMA)
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform_u24(count: int, seed: int, start: int = 0) -> np.ndarray:
+    """count integers k in [0, 2^24): the top 24 bits of mix64(seed<<32 + j)."""
+    j = np.arange(start, start + count, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = mix64((np.uint64(seed) << np.uint64(32)) + j)
+    return (z >> np.uint64(40)).astype(np.int64)
+
+
+def uniform_points(P: int, d: int, seed: int = 42, dtype=np.float64) -> np.ndarray:
+    """[P][d] row-major points k * 2^-24 (exact in FP32 and FP64)."""
+    k = uniform_u24(P * d, seed)
+    return (k.astype(np.float64) * 2.0 ** -24).reshape(P, d).astype(dtype)
+
+
+# --- test functions (FP64 unless stated) -------------------------------------------
+
+def f_prod_parabola(x: np.ndarray) -> np.ndarray:
+    """prod_i 4 x_i (1 - x_i): hierarchical surpluses are exactly 4^-|l|_1 (0-based l)."""
+    return np.prod(4.0 * x * (1.0 - x), axis=1)
+
+
+def f_gauss(x: np.ndarray) -> np.ndarray:
+    """exp(-sum_i (x_i - 1/2)^2), the d=10, n=8 drop-in workload."""
+    return np.exp(-np.sum((x - 0.5) ** 2, axis=1))
+
+
+def f_reference(x: np.ndarray) -> np.ndarray:
+    """sum_i x_i^2 (1 - x_i)^2, the reference benchmark's own function (asg.c:87-94)."""
+    return np.sum(x * x * (1.0 - x) * (1.0 - x), axis=1)
+
+
+def gaussian_sum_params(d: int, count: int = 8, seed: int = 7):
+    """Centres uniform in [0.2, 0.8]^d, widths log-uniform in [0.05, 0.5], unit weights."""
+    u = uniform_u24(count * d * 2, seed).astype(np.float64) * 2.0 ** -24
+    c = 0.2 + 0.6 * u[: count * d].reshape(count, d)
+    w = np.exp(np.log(0.05) + (np.log(0.5) - np.log(0.05)) * u[count * d:].reshape(count, d))
+    return c, w
+
+
+def f_gaussian_sum(x: np.ndarray, centers: np.ndarray, widths: np.ndarray) -> np.ndarray:
+    """sum_g prod_k exp(-(x_k - c_gk)^2 / (2 s_gk^2)), the d=20, n=6 precision workload."""
+    out = np.zeros(x.shape[0])
+    for c, s in zip(centers, widths):
+        out += np.exp(-np.sum((x - c) ** 2 / (2.0 * s * s), axis=1))
+    return out
+
+
+def f_sin_prod(x: np.ndarray, c: float) -> np.ndarray:
+    """prod_i sin(pi x_i) * (1 + 0.1 c), the batched-grid workload."""
+    return np.prod(np.sin(np.pi * x), axis=1) * (1.0 + 0.1 * c)

@@ -1,0 +1,153 @@
+/*
+ * sgrid.h — C ABI of the MI355X compact sparse grid library (libsgrid.so).
+ *
+ * The library holds a compact sparse grid (no boundary points, 0-based levels) in
+ * HBM and runs the reference benchmark's hot path on one AMD Instinct MI355X
+ * (gfx950): the offset <-> (levels, indices) bijection, hierarchization and
+ * evaluation.  Each entry point cites the reference interface it replaces; paths are
+ * relative to the reference repository adeftu/adaptive_sparse_grid_bench:
+ *   asg.c  = adaptive_sparse_grid_bench.c
+ *   asg.cu = adaptive_sparse_grid_bench_cuda.cu
+ *
+ * Conventions
+ *   - Every function returns an int status: SG_OK (0) or a negative SG_E_* code.
+ *     sg_last_error() returns a thread-local message for the last failure.
+ *   - d_* arguments are device pointers on the device that was current when the
+ *     grid was created; other pointers are host memory.  Host buffers stay owned by
+ *     the caller, device tables by the grid handle.
+ *   - stream is a hipStream_t passed as void* (NULL = the null stream).  Device calls
+ *     are asynchronous on that stream and perform no device-wide synchronisation.
+ *   - Grid values use the reference's compact order (asg.c:187-243): level sum
+ *     ascending, then rank within the level sum (asg.c:266-279), then the row-major
+ *     in-block index with dimension 0 most significant.  Query points are row-major
+ *     [P][d] (asg.c:1259) with 0 <= x < 1.
+ *   - A handle may be used from several host threads at once: after creation the
+ *     calls only read it.
+ */
+#ifndef SGRID_H
+#define SGRID_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_OK 0
+#define SG_E_INVALID (-1)     /* bad argument or precondition */
+#define SG_E_NOMEM (-2)       /* device or host allocation failed */
+#define SG_E_HIP (-3)         /* HIP runtime error */
+#define SG_E_RANGE (-4)       /* offset, level or size out of range */
+#define SG_E_UNSUPPORTED (-5) /* shape outside the kernels' limits */
+
+#define SG_MAX_DIMS 20        /* asg.cu:17 */
+
+typedef struct sg_grid sg_grid;
+
+const char *sg_last_error(void);
+const char *sg_status_string(int status);
+int sg_version(void);
+
+/* ------------------------------------------------------------------ grids ---- */
+
+/* Restricted-regular grid { l : |l|_1 <= num_levels-1, l_k <= limits[k] } with its
+ * shape tables and launch tables, uploaded to the current device.
+ * Replaces init() on the globals num_dims/num_levels/limits (asg.c:101-184) and the
+ * CUDA host set-up init_restr_vec/init_dp_mat/init_lidx_vect (asg.cu:72-102).
+ * limits == NULL means num_levels-1 in every dimension (asg.c:1760-1763). */
+int sg_grid_create(int num_dims, int num_levels, const int *limits, sg_grid **out);
+
+/* Refined-subspace grid: an explicit downward-closed set of `count` level vectors
+ * (levels[count][num_dims], 0-based, each <= 15).  Generalises the restriction vector
+ * (asg.c:82-83); a restricted-regular set given this way gets the identical order. */
+int sg_grid_create_explicit(int num_dims, const unsigned char *levels, long long count,
+                            sg_grid **out);
+
+int sg_grid_destroy(sg_grid *grid);
+
+/* num_dims, num_levels (level sums 0..num_levels-1), N and S (asg.c:131-140). */
+int sg_grid_info(const sg_grid *grid, int *num_dims, int *num_levels,
+                 long long *num_grid_points, long long *num_subspaces);
+
+/* dp matrix [d][n] (restricted-regular grids only; NULL to skip) and the level-sum
+ * start offsets lidx[n+1] (asg.c:108-140 dp_mat, lidx_vect). */
+int sg_grid_tables(const sg_grid *grid, long long *dp, long long *lidx);
+
+/* level vectors [S][d] and base offsets [S] of the subspaces in compact order. */
+int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *base);
+
+/* Tuning of the launches: threads per workgroup of the evaluate kernels (64, 128 or
+ * 256) and the target values per hierarchization tile (>= 64).  0 keeps a value. */
+int sg_grid_set_tuning(sg_grid *grid, int eval_block, int hier_tile_values);
+
+/* ------------------------------------------------------ host bijection ------- */
+
+int sg_gp2idx(const sg_grid *grid, const int *levels, const int *indices,
+              long long *index);                                 /* asg.c:187-207 */
+int sg_idx2gp(const sg_grid *grid, long long index, int *levels,
+              int *indices);                                     /* asg.c:210-243 */
+int sg_idx2l(const sg_grid *grid, long long rank, int level_sum,
+             int *levels);                                       /* asg.c:245-263 */
+int sg_l2idx(const sg_grid *grid, const int *levels, long long *rank); /* asg.c:266-279 */
+
+/* ---------------------------------------------------- device bijection ------- */
+
+/* idx2gp for `count` offsets (d_index == NULL: offsets 0..count-1) into row-major
+ * [count][d] int32 outputs; an offset outside [0, N) yields -1 entries.
+ * Device form of asg.cu:218-259 (idx2gp). */
+int sg_idx2gp_dev(const sg_grid *grid, const int *d_index, long long count, int *d_levels,
+                  int *d_indices, void *stream);
+
+/* gp2idx for `count` (levels, indices) rows; a point not in the grid yields -1.
+ * Device form of asg.cu:189-214 (gp2idx). */
+int sg_gp2idx_dev(const sg_grid *grid, const int *d_levels, const int *d_indices,
+                  long long count, int *d_index, void *stream);
+
+/* Coordinates (2 i_k + 1) / 2^(l_k + 1) of every grid point, [N][d] FP64
+ * (asg.c:143-152, asg.cu:105-117). */
+int sg_grid_coords_f64(const sg_grid *grid, double *d_coords, void *stream);
+
+/* ------------------------------------------------------- hierarchize -------- */
+
+/* Nodal values -> hierarchical surpluses in place, d_values[N].  One sweep per
+ * dimension, v -= (left parent + right parent) / 2, parents read before their own
+ * update.  Bit-identical to hierarchize0 (asg.c:557-571) and every other CPU variant
+ * (asg.c:1810-1928); replaces hierarchize0..8 / omp_hierarchize3 (asg.c:557-1233) and
+ * the CUDA launches hierarchize0..7 (asg.cu:448-1160, 1720-1806). */
+int sg_hierarchize_f64(const sg_grid *grid, double *d_values, void *stream);
+int sg_hierarchize_f32(const sg_grid *grid, float *d_values, void *stream);
+
+/* --------------------------------------------------------- evaluate --------- */
+
+/* Interpolant at P points: d_out[p] = sum over subspaces of prod_k hat_k * surplus.
+ * Replaces evaluate0..5 / omp_evaluate2/5 (asg.c:1237-1722; out overwritten as in
+ * asg.c:1243-1244) and the CUDA evaluate0..5 (asg.cu:1169-1598).
+ *   _f64   FP64, depth-first walk with incremental hat products; agrees with
+ *          evaluate0's value within 1e-12 relative to sum |terms|.
+ *   _f32c  FP32 surpluses and points, Kahan-compensated accumulation.
+ *   _ref_* the reference's exact operation order: bit-identical to evaluate0 in the
+ *          same precision (slower; what the asg_compat shims call). */
+int sg_evaluate_f64(const sg_grid *grid, const double *d_alpha, const double *d_coords,
+                    long long num_evals, double *d_out, void *stream);
+int sg_evaluate_f32c(const sg_grid *grid, const float *d_alpha, const float *d_coords,
+                     long long num_evals, float *d_out, void *stream);
+int sg_evaluate_ref_f64(const sg_grid *grid, const double *d_alpha, const double *d_coords,
+                        long long num_evals, double *d_out, void *stream);
+int sg_evaluate_ref_f32(const sg_grid *grid, const float *d_alpha, const float *d_coords,
+                        long long num_evals, float *d_out, void *stream);
+
+/* ----------------------------------------------------------- inputs --------- */
+
+/* count values k * 2^-24, k = top 24 bits of splitmix64(seed * 2^32 + j), j = 0..count-1:
+ * seeded uniform query coordinates exact in FP32 and FP64 (replaces the time-seeded
+ * init_coord_mat, asg.cu:152-165). */
+int sg_uniform_points_f64(long long count, unsigned long long seed, double *d_out,
+                          void *stream);
+int sg_uniform_points_f32(long long count, unsigned long long seed, float *d_out,
+                          void *stream);
+/* FP64 -> FP32 rounding copy (surpluses for the FP32 evaluate). */
+int sg_convert_f64_to_f32(const double *d_in, float *d_out, long long count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGRID_H */

@@ -61,6 +61,46 @@ int finish_create(sg_grid* h, sg_grid** out) {
     return sg::kOk;
 }
 
+// Launches of at least tune.bin_min_points points first bin the points spatially (a
+// permutation in stream-ordered scratch memory, freed on the same stream), then walk
+// them in binned order; every point's arithmetic is the same either way.
+int eval_one(const sg_grid* h, const double* a, const double* x, int64_t P, double* o,
+             const uint32_t* perm, hipStream_t s) {
+    return sg::launch_evaluate_f64(h->g, h->t, a, x, P, o, h->tune, perm, s);
+}
+int eval_one(const sg_grid* h, const float* a, const float* x, int64_t P, float* o,
+             const uint32_t* perm, hipStream_t s) {
+    return sg::launch_evaluate_f32c(h->g, h->t, a, x, P, o, h->tune, perm, s);
+}
+int sort_one(const double* x, int64_t P, int d, int kb, void* ws, uint32_t* perm, hipStream_t s) {
+    return sg::launch_sort_points_f64(x, P, d, kb, ws, perm, s);
+}
+int sort_one(const float* x, int64_t P, int d, int kb, void* ws, uint32_t* perm, hipStream_t s) {
+    return sg::launch_sort_points_f32(x, P, d, kb, ws, perm, s);
+}
+
+template <class T>
+int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, void* stream) {
+    hipStream_t s = as_stream(stream);
+    // binning pays when the walk per point is long: skip it for small grids
+    if (P < h->tune.bin_min_points || h->tune.bin_min_points <= 0 || h->g.S < 1024)
+        return eval_one(h, a, x, P, o, nullptr, s);
+    const int kb = sg::key_bits_for(P, h->g.d);
+    const size_t ws_bytes = sg::sort_workspace_bytes(P, kb);
+    void* mem = nullptr;
+    hipError_t e = hipMallocAsync(&mem, ws_bytes + size_t(P) * 4, s);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? sg::kNoMem : sg::kHip,
+                    std::string("evaluate: hipMallocAsync: ") + hipGetErrorString(e));
+    uint32_t* perm = reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + ws_bytes);
+    int st = sort_one(x, P, h->g.d, kb, mem, perm, s);
+    if (st == sg::kOk) st = eval_one(h, a, x, P, o, perm, s);
+    e = hipFreeAsync(mem, s);
+    if (st == sg::kOk && e != hipSuccess)
+        return fail(sg::kHip, std::string("evaluate: hipFreeAsync: ") + hipGetErrorString(e));
+    return st;
+}
+
 }  // namespace
 
 extern "C" {
@@ -133,14 +173,19 @@ int sg_grid_subspaces(const sg_grid* grid, unsigned char* levels, long long* bas
     return sg::kOk;
 }
 
-int sg_grid_set_tuning(sg_grid* grid, int eval_block, int hier_tile_values) {
+int sg_grid_set_tuning(sg_grid* grid, int eval_block, int eval_variant, int bin_min_points,
+                       int hier_tile_values) {
     if (!grid) return fail(sg::kInvalid, "sg_grid_set_tuning: NULL grid");
     if (eval_block != 0 && eval_block != 64 && eval_block != 128 && eval_block != 256)
         return fail(sg::kInvalid, "sg_grid_set_tuning: eval_block must be 64, 128 or 256");
+    if (eval_variant < -1 || eval_variant >= SG_EVAL_VARIANTS)
+        return fail(sg::kInvalid, "sg_grid_set_tuning: unknown eval_variant");
     if (hier_tile_values != 0 && hier_tile_values < 64)
         return fail(sg::kInvalid, "sg_grid_set_tuning: hier_tile_values must be >= 64");
     std::lock_guard<std::mutex> lock(grid->mu);
     if (eval_block) grid->tune.block = eval_block;
+    if (eval_variant >= 0) grid->tune.variant = eval_variant;
+    if (bin_min_points) grid->tune.bin_min_points = bin_min_points < 0 ? 0 : bin_min_points;
     if (hier_tile_values && hier_tile_values != grid->hier_tile_values) {
         grid->hier_tile_values = hier_tile_values;
         int st = sg::build_launch_tables(grid->g, hier_tile_values);
@@ -235,15 +280,13 @@ int sg_hierarchize_f32(const sg_grid* grid, float* d_values, void* stream) {
 int sg_evaluate_f64(const sg_grid* grid, const double* d_alpha, const double* d_coords,
                     long long num_evals, double* d_out, void* stream) {
     SG_EVAL_CHECK("sg_evaluate_f64")
-    return sg::launch_evaluate_f64(grid->g, grid->t, d_alpha, d_coords, num_evals, d_out,
-                                   grid->tune, as_stream(stream));
+    return evaluate_binned<double>(grid, d_alpha, d_coords, num_evals, d_out, stream);
 }
 
 int sg_evaluate_f32c(const sg_grid* grid, const float* d_alpha, const float* d_coords,
                      long long num_evals, float* d_out, void* stream) {
     SG_EVAL_CHECK("sg_evaluate_f32c")
-    return sg::launch_evaluate_f32c(grid->g, grid->t, d_alpha, d_coords, num_evals, d_out,
-                                    grid->tune, as_stream(stream));
+    return evaluate_binned<float>(grid, d_alpha, d_coords, num_evals, d_out, stream);
 }
 
 int sg_evaluate_ref_f64(const sg_grid* grid, const double* d_alpha, const double* d_coords,

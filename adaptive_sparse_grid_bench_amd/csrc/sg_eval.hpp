@@ -1,0 +1,418 @@
+// Evaluation kernel: the interpolant u(x) = sum over subspaces l of prod_k hat_k(l_k) *
+// surplus[base(l) + in-block index], one query point per lane (values of
+// adaptive_sparse_grid_bench.c "asg.c":1237-1274).
+//
+// Instead of recomputing the d-term hat product for every subspace, each lane walks the
+// level set depth first (dimension d-1 outermost, dimension 0 innermost) and keeps
+// prefix products, so a leaf costs one multiply, one shift-add and one FMA.  Leaf
+// base offsets come from a table in walk order, read through the scalar cache (the
+// walk is uniform across the wave).  The J innermost dimensions keep hats and offsets
+// for every level in register tables; the leaves below a depth-J node then form a
+// static block whose gathers are issued in batches before any is consumed.
+//
+// Template parameters:
+//   D    dimensions;  LT  levels per register table (8) or 16 (no tables);
+//   J    tabled dimensions;  B  gathers per batch;  W  waves per SIMD the register
+//        allocation targets (0 = compiler's choice);
+//   T    coordinate / surplus type;  Acc  accumulator (FP64 FMA or FP32 Kahan);
+//   EXPL explicit (refined-subspace) level set: node bounds come from a byte stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sg_grid.hpp"
+
+namespace sg {
+namespace eval {
+
+constexpr int kThreads = 256;
+
+template <class T>
+struct Params {
+    const T* alpha;
+    const T* coords;            // [P][D] row-major
+    T* out;
+    int64_t P;
+    const int32_t* leaf_base;   // subspace base offsets in walk order
+    const uint8_t* bounds;      // explicit sets: one code per walk node
+    const uint32_t* perm;       // lane order of the points (spatially binned) or null
+    uint32_t alpha_bytes;       // N * sizeof(T): the buffer range of the surpluses
+    int n;                      // level sums 0..n-1
+    int limits[kMaxDims];
+};
+
+struct AccF64 {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double w, double a) { s = fma(w, a, s); }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
+// Kahan-compensated FP32 accumulation across subspaces
+struct AccKahan {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float w, float a) {
+        const float y = fmaf(w, a, -c);
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ float value() const { return s; }
+};
+
+// Hat values and in-block byte offsets of one coordinate at levels 0..LT-1.
+// With m_l = x 2^(l+1) - (2 t_l + 1), t_l = floor(x 2^l):  hat_l = 1 - |m_l|,
+// t_{l+1} = 2 t_l + [m_l >= 0],  m_{l+1} = 2 m_l - (m_l >= 0 ? 1 : -1).  For x on the
+// 2^-24 lattice every step is exact, so hat_l equals asg.c:1257-1262's value.
+template <int LT, class T>
+__device__ __forceinline__ void level_table(T x, T (&ph)[LT], uint32_t (&ub)[LT]) {
+    T m = T(2) * x - T(1);
+    uint32_t t = 0;
+#pragma unroll
+    for (int l = 0; l < LT; ++l) {
+        ph[l] = T(1) - fabs(m);
+        ub[l] = t * uint32_t(sizeof(T));
+        const bool ge = m >= T(0);
+        t = 2u * t + (ge ? 1u : 0u);
+        m = T(2) * m + (ge ? T(-1) : T(1));
+    }
+}
+
+// Surplus gathers are raw buffer loads: the subspace's base offset rides in the
+// instruction's scalar offset and the lane's in-block byte offset in its vector offset,
+// so a leaf needs no 64-bit address arithmetic; reads past N return 0.
+__device__ __forceinline__ double gather(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                         double*) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float gather(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                        float*) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+template <class T>
+__device__ __forceinline__ T gather_leaf(__amdgpu_buffer_rsrc_t r, int32_t base, uint32_t voff) {
+    return gather(r, voff, uint32_t(base) * uint32_t(sizeof(T)), static_cast<T*>(nullptr));
+}
+
+template <class T, class Acc>
+struct State {
+    __amdgpu_buffer_rsrc_t alpha;
+    const int32_t* leaf_base;
+    const uint8_t* bounds;
+    int q;                      // next leaf (uniform)
+    int bq;                     // next node code (uniform; explicit sets)
+    Acc acc;
+    __device__ __forceinline__ void leaf(T w, uint32_t voff) {
+        acc.add(w, gather_leaf<T>(alpha, leaf_base[q], voff));
+        ++q;
+    }
+};
+
+// Per-lane state: m0 = 2x - 1 per dimension for the incremental walk, and the hat /
+// offset tables of the J innermost dimensions.
+template <int D, int LT, int J, class T>
+struct Lane {
+    static constexpr int kJ = J;
+    T m0[D];
+    T ph[J > 0 ? J : 1][J > 0 ? LT : 1];
+    uint32_t ub[J > 0 ? J : 1][J > 0 ? LT : 1];
+};
+
+// The level vectors (l_{J-1}, ..., l_0) with sum <= R in walk order (dim J-1
+// outermost), built at compile time.
+__host__ __device__ constexpr int binom(int a, int b) {
+    int r = 1;
+    for (int i = 1; i <= b; ++i) r = r * (a - b + i) / i;
+    return r;
+}
+
+template <int J, int R>
+struct Simplex {
+    static constexpr int C = binom(R + J, J);
+    int l[C][J];
+    __host__ __device__ constexpr Simplex() : l{} {
+        int v[J] = {};
+        for (int c = 0; c < C; ++c) {
+            for (int j = 0; j < J; ++j) l[c][j] = v[j];
+            for (int j = 0; j < J; ++j) {      // odometer, dim 0 fastest
+                ++v[j];
+                int sum = 0;
+                for (int k = 0; k < J; ++k) sum += v[k];
+                if (sum <= R) break;
+                v[j] = 0;
+            }
+        }
+    }
+};
+
+template <int J, int R, int c, int j>
+__host__ __device__ constexpr int simplex_level() {
+    return Simplex<J, R>().l[c][j];
+}
+
+// in-block byte offset of leaf c: dims J-1 .. 0, each shifted by the levels above it
+template <int J, int R, int c, int j, class Ln>
+__device__ __forceinline__ uint32_t leaf_offset(const Ln& ln, uint32_t off, int sh) {
+    constexpr int l = simplex_level<J, R, c, j>();
+    off += ln.ub[j][l] << sh;
+    if constexpr (j > 0) return leaf_offset<J, R, c, j - 1>(ln, off, sh + l);
+    return off;
+}
+
+template <int J, int R, int c, int j, class Ln, class T>
+__device__ __forceinline__ T leaf_weight(const Ln& ln, T w) {
+    constexpr int l = simplex_level<J, R, c, j>();
+    w = w * ln.ph[j][l];
+    if constexpr (j > 0) return leaf_weight<J, R, c, j - 1>(ln, w);
+    return w;
+}
+
+template <int J, int R, int B, int c, int cend, class Ln, class T>
+__device__ __forceinline__ void block_issue(T (&a)[B], const Ln& ln, uint32_t off, int hsum,
+                                            const int32_t* bases, __amdgpu_buffer_rsrc_t alpha) {
+    if constexpr (c < cend) {
+        a[c % B] = gather_leaf<T>(alpha, bases[c], leaf_offset<J, R, c, J - 1>(ln, off, hsum));
+        block_issue<J, R, B, c + 1, cend>(a, ln, off, hsum, bases, alpha);
+    }
+}
+
+template <int J, int R, int B, int c, int cend, class Ln, class T, class Acc>
+__device__ __forceinline__ void block_consume(const T (&a)[B], const Ln& ln, T pp, Acc& acc) {
+    if constexpr (c < cend) {
+        acc.add(leaf_weight<J, R, c, J - 1>(ln, pp), a[c % B]);
+        block_consume<J, R, B, c + 1, cend>(a, ln, pp, acc);
+    }
+}
+
+template <int J, int R, int B, int c0, class Ln, class T, class Acc>
+__device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, int hsum,
+                                              const int32_t* bases, State<T, Acc>& st) {
+    constexpr int C = Simplex<J, R>::C;
+    if constexpr (c0 < C) {
+        constexpr int cend = (c0 + B < C) ? c0 + B : C;
+        T a[B];
+        block_issue<J, R, B, c0, cend>(a, ln, off, hsum, bases, st.alpha);
+        block_consume<J, R, B, c0, cend>(a, ln, pp, st.acc);
+        block_batches<J, R, B, cend>(ln, pp, off, hsum, bases, st);
+    }
+}
+
+// All C(R+J, J) leaves below a depth-J node whose remaining level budget R fits inside
+// the J tabled dimensions.
+template <int J, int R, int B, class Ln, class T, class Acc>
+__device__ __forceinline__ void static_block(const Ln& ln, T pp, uint32_t off, int hsum,
+                                             State<T, Acc>& st) {
+    block_batches<J, R, B, 0>(ln, pp, off, hsum, st.leaf_base + st.q, st);
+    st.q += Simplex<J, R>::C;
+}
+
+template <int J, int LT, int B, class Ln, class T, class Acc>
+__device__ __forceinline__ void static_block_dispatch(int R, const Ln& ln, T pp, uint32_t off,
+                                                      int hsum, State<T, Acc>& st) {
+    switch (R) {
+#define SG_BLK(RR) \
+    case RR: if constexpr (RR < LT) static_block<J, RR, B>(ln, pp, off, hsum, st); break;
+        SG_BLK(0) SG_BLK(1) SG_BLK(2) SG_BLK(3) SG_BLK(4) SG_BLK(5) SG_BLK(6) SG_BLK(7)
+#undef SG_BLK
+        default: break;
+    }
+}
+
+// Depth-first walk.  Dimension K's level runs 0..bound, where bound = min(rem,
+// limit_K) for a restricted-regular set and comes from the node-code stream for an
+// explicit set.  Dimension d-1 is outermost, so for a fixed level sum the leaves come
+// in rank order.  pp is the product of the hats of dims > K, off the in-block byte
+// offset they contribute, hsum the sum of their levels (the shift of dimension K's
+// index bits), rem the level-sum budget left.
+//
+// Node codes of an explicit set (one byte per visited node, walk order): at depth J-1
+// a code below 0x80 is a full static block of budget R = code; otherwise code & 0x7f is
+// the largest l_K under this node.
+template <int K, int D, int LT, int J, int B, bool EXPL, class T, class Acc>
+struct Walker {
+    using Ln = Lane<D, LT, J, T>;
+    static __device__ __forceinline__ void run(const Ln& ln, const int* lim, T pp, uint32_t off,
+                                               int hsum, int rem, State<T, Acc>& st) {
+        int b;
+        if constexpr (EXPL) {
+            const int code = st.bounds[st.bq++];
+            if constexpr (J > 0 && K == J - 1) {
+                if (code < 0x80) {
+                    static_block_dispatch<J, LT, B>(code, ln, pp, off, hsum, st);
+                    return;
+                }
+            }
+            b = code & 0x7f;
+        } else {
+            if constexpr (J > 0 && K == J - 1) {
+                bool fits = true;
+#pragma unroll
+                for (int j = 0; j < J; ++j) fits = fits && rem <= lim[j];
+                if (fits) {
+                    static_block_dispatch<J, LT, B>(rem, ln, pp, off, hsum, st);
+                    return;
+                }
+            }
+            b = min(rem, lim[K]);
+        }
+        // walk dimension K incrementally (outer dims, or a block cut by a restriction)
+        T m = ln.m0[K];
+        uint32_t t = 0;
+        for (int l = 0; l <= b; ++l) {
+            const T phi = T(1) - fabs(m);
+            const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
+            if constexpr (K == 0) st.leaf(pp * phi, o);
+            else Walker<K - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, lim, pp * phi, o, hsum + l,
+                                                                rem - l, st);
+            const bool ge = m >= T(0);
+            t = 2u * t + (ge ? 1u : 0u);
+            m = T(2) * m + (ge ? T(-1) : T(1));
+        }
+    }
+};
+
+template <int D, int LT, int J, int B, int W, bool EXPL, class T, class Acc>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
+void k_eval(Params<T> prm) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool live = p < prm.P;
+    // idle lanes walk point 0 and store nothing; with a permutation, neighbouring lanes
+    // hold spatially close points, so a wave's gathers share cache lines
+    const int64_t pc = live ? (prm.perm ? int64_t(prm.perm[p]) : p) : 0;
+    Lane<D, LT, J, T> ln;
+    const T* x = prm.coords + pc * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const T xk = x[k];
+        if constexpr (J > 0) {
+            if (k < J) level_table<LT, T>(xk, ln.ph[k < J ? k : 0], ln.ub[k < J ? k : 0]);
+        }
+        ln.m0[k] = T(2) * xk - T(1);
+    }
+    // buffer descriptor word 3 for gfx9-family raw (untyped) buffer access
+    State<T, Acc> st{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(prm.alpha), short(0),
+                                                       int(prm.alpha_bytes), 0x00020000),
+                     prm.leaf_base, prm.bounds, 0, 0, Acc()};
+    Walker<D - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, prm.limits, T(1), 0u, 0, prm.n - 1, st);
+    if (live) prm.out[pc] = st.acc.value();
+}
+
+// Compiled variants of the register-table walk (LT = 8); the autotuner picks one per
+// grid shape and precision.  Tables for J dimensions cost 3 J LT registers (FP64);
+// larger batches keep more gathers in flight per wave, fewer registers more waves.
+struct Variant {
+    int J, B, W;
+};
+constexpr Variant kVariants[] = {{3, 8, 3}, {2, 8, 3}, {3, 4, 4}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+// tabled dimensions the explicit-set kernels use (their node-code stream depends on it)
+constexpr int kExplicitJ = kExplicitTabDims;
+
+template <int D, int LT, int J, int B, int W, bool EXPL, class T, class Acc>
+hipError_t launch_one(const Params<T>& prm, int block, hipStream_t s) {
+    const int64_t blocks = (prm.P + block - 1) / block;
+    constexpr int Jd = J < D ? J : D;
+    hipLaunchKernelGGL((k_eval<D, LT, Jd, B, W, EXPL, T, Acc>), dim3(unsigned(blocks)),
+                       dim3(block), 0, s, prm);
+    return hipGetLastError();
+}
+
+// variant index -> (J, B, W) with LT = 8, or the table-free walk with LT = 16.
+// Instantiated per dimension range in sg_eval_inst.hip (one object per range and
+// precision, compiled in parallel).
+template <int D, bool EXPL, class T, class Acc>
+hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int block, hipStream_t s)
+#ifdef SG_EVAL_INSTANTIATE
+{
+    if (lt > 8) return launch_one<D, 16, 0, 8, 0, EXPL, T, Acc>(prm, block, s);
+    if constexpr (EXPL) {
+        return launch_one<D, 8, kExplicitJ, 8, 3, true, T, Acc>(prm, block, s);
+    } else {
+        switch (variant) {
+            case 1: return launch_one<D, 8, 2, 8, 3, false, T, Acc>(prm, block, s);
+            case 2: return launch_one<D, 8, 3, 4, 4, false, T, Acc>(prm, block, s);
+            default: return launch_one<D, 8, 3, 8, 3, false, T, Acc>(prm, block, s);
+        }
+    }
+}
+#else
+;
+#endif
+
+template <bool EXPL, class T, class Acc>
+hipError_t launch_d(int d, int variant, int lt, const Params<T>& prm, int block, hipStream_t s) {
+    switch (d) {
+#define SG_D(DD) \
+    case DD: return launch_variant<DD, EXPL, T, Acc>(variant, lt, prm, block, s);
+        SG_D(1) SG_D(2) SG_D(3) SG_D(4) SG_D(5) SG_D(6) SG_D(7) SG_D(8) SG_D(9) SG_D(10)
+        SG_D(11) SG_D(12) SG_D(13) SG_D(14) SG_D(15) SG_D(16) SG_D(17) SG_D(18) SG_D(19)
+        SG_D(20)
+#undef SG_D
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace eval
+}  // namespace sg
+
+#include <string>
+
+#include "sg_kernels.hpp"
+
+namespace sg {
+namespace eval {
+
+template <class T, class Acc>
+int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const T* coords,
+                    int64_t P, T* out, const EvalTuning& tu, const uint32_t* perm,
+                    hipStream_t s) {
+    if (P == 0) return kOk;
+    if (tu.block != 64 && tu.block != 128 && tu.block != 256) {
+        set_error("evaluate: block must be 64, 128 or 256");
+        return kInvalid;
+    }
+    if (tu.variant < 0 || tu.variant >= kNumVariants) {
+        set_error("evaluate: unknown kernel variant");
+        return kInvalid;
+    }
+    if (g.N * int64_t(sizeof(T)) > int64_t(0x7fffffff)) {
+        set_error("evaluate: surpluses beyond 2 GiB");
+        return kRange;
+    }
+    if ((P + tu.block - 1) / tu.block > int64_t(0x7fffffff)) {
+        set_error("evaluate: too many points for one launch");
+        return kRange;
+    }
+    Params<T> prm{};
+    prm.alpha = alpha;
+    prm.coords = coords;
+    prm.out = out;
+    prm.P = P;
+    prm.leaf_base = t.dfs_base;
+    prm.bounds = t.dfs_bound;
+    prm.perm = perm;
+    prm.alpha_bytes = uint32_t(g.N * int64_t(sizeof(T)));
+    prm.n = g.n;
+    for (int j = 0; j < g.d; ++j) prm.limits[j] = g.limits[j];
+    int lt = 8;
+    hipError_t e;
+    if (g.explicit_set) {
+        if (explicit_tab_dims(g) == 0) lt = 16;   // no tables: no blocks in the codes
+        e = launch_d<true, T, Acc>(g.d, 0, lt, prm, tu.block, s);
+    } else {
+        const int J = kVariants[tu.variant].J < g.d ? kVariants[tu.variant].J : g.d;
+        for (int j = 0; j < J; ++j)
+            if (g.limits[j] > 7) lt = 16;
+        e = launch_d<false, T, Acc>(g.d, tu.variant, lt, prm, tu.block, s);
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("k_eval launch: ") + hipGetErrorString(e));
+        return kHip;
+    }
+    return kOk;
+}
+
+}  // namespace eval
+}  // namespace sg

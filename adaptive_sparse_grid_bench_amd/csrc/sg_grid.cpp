@@ -275,41 +275,108 @@ namespace {
 // Depth-first walk over the level set with dimension d-1 outermost and dimension 0
 // innermost; visit(levels) is called at every leaf.  For a fixed level sum the leaves
 // come in rank order, because rank order is lexicographic from dimension d-1 down.
-template <class F>
-void dfs_walk(const Grid& g, F&& visit, std::vector<uint8_t>* bounds) {
+// For an explicit set it also writes the node-code stream the evaluation kernel reads
+// (one byte per visited node): at depth J-1 a code below 0x80 marks a full block
+// { sum of l_{J-1..0} <= R } and is R; otherwise (code & 0x7f) is the node's bound.
+struct DfsWalk {
+    const Grid& g;
+    int J;
+    std::vector<uint8_t>* codes;
+    std::vector<int32_t>* leaves;
     int lv[kMaxDims] = {};
-    int d = g.d;
-    // largest admissible l_k given the levels already fixed above k (and zeros below)
-    auto bound_at = [&](int k, int rem) {
+    bool ok = true;
+
+    // largest admissible l_k given the levels fixed above k (and zeros below)
+    int bound_at(int k, int rem) {
         if (!g.explicit_set) return std::min(rem, g.limits[k]);
+        for (int j = 0; j < k; ++j) lv[j] = 0;
         int b = -1;
         for (int l = 0; l <= std::min(rem, g.limits[k]); ++l) {
             lv[k] = l;
-            for (int j = 0; j < k; ++j) lv[j] = 0;
             if (subspace_id(g, lv) < 0) break;
             b = l;
         }
         lv[k] = 0;
         return b;
-    };
-    struct Rec {
-        static void go(const Grid& g, int k, int rem, int* lv, F& visit,
-                       std::vector<uint8_t>* bounds, decltype(bound_at)& bnd) {
-            int b = bnd(k, rem);
-            if (bounds) bounds->push_back(static_cast<uint8_t>(b));
-            for (int l = 0; l <= b; ++l) {
-                lv[k] = l;
-                if (k == 0) visit(lv);
-                else go(g, k - 1, rem - l, lv, visit, bounds, bnd);
+    }
+
+    // does the subtree under the current prefix hold exactly { sum(l_0..l_k) <= R }?
+    int full_block_budget(int k, int rem) {
+        int R = -1;
+        int64_t count = 0;
+        int v[kMaxDims] = {};
+        const int J1 = k + 1;
+        // enumerate the candidates l_k..l_0 with sum <= rem (odometer, dim 0 fastest)
+        while (true) {
+            int sum = 0;
+            for (int j = 0; j < J1; ++j) { lv[j] = v[j]; sum += v[j]; }
+            if (subspace_id(g, lv) >= 0) {
+                ++count;
+                R = std::max(R, sum);
             }
-            lv[k] = 0;
+            int j = 0;
+            for (; j < J1; ++j) {
+                ++v[j];
+                int s2 = 0;
+                for (int q = 0; q < J1; ++q) s2 += v[q];
+                if (s2 <= rem) break;
+                v[j] = 0;
+            }
+            if (j == J1) break;
         }
-    };
-    (void)d;
-    Rec::go(g, g.d - 1, g.n - 1, lv, visit, bounds, bound_at);
-}
+        for (int j = 0; j < J1; ++j) lv[j] = 0;
+        if (R < 0 || R > 7) return -1;
+        int64_t full = 1;
+        for (int i = 1; i <= J1; ++i) full = full * (R + i) / i;   // C(R + J, J)
+        return count == full ? R : -1;
+    }
+
+    void leaf() {
+        int64_t id = subspace_id(g, lv);
+        if (id < 0) { ok = false; return; }
+        leaves->push_back(g.sub_base[id]);
+    }
+
+    void go(int k, int rem) {
+        int b = bound_at(k, rem);
+        if (codes && J > 0 && k == J - 1) {
+            int R = full_block_budget(k, rem);
+            if (R >= 0) {
+                codes->push_back(static_cast<uint8_t>(R));
+                block(k, R);
+                return;
+            }
+            codes->push_back(static_cast<uint8_t>(0x80 | b));
+        } else if (codes) {
+            codes->push_back(static_cast<uint8_t>(b));
+        }
+        for (int l = 0; l <= b; ++l) {
+            lv[k] = l;
+            if (k == 0) leaf();
+            else go(k - 1, rem - l);
+        }
+        lv[k] = 0;
+    }
+
+    // leaves of a full block in walk order
+    void block(int k, int R) {
+        for (int l = 0; l <= R; ++l) {
+            lv[k] = l;
+            if (k == 0) leaf();
+            else block(k - 1, R - l);
+        }
+        lv[k] = 0;
+    }
+};
 
 }  // namespace
+
+int explicit_tab_dims(const Grid& g) {
+    const int J = std::min(g.d, kExplicitTabDims);
+    for (int j = 0; j < J; ++j)
+        if (g.max_level[j] > 7) return 0;
+    return J;
+}
 
 int build_launch_tables(Grid& g, int hier_tile_values) {
     const int d = g.d;
@@ -317,15 +384,10 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
     g.dfs_base.clear();
     g.dfs_base.reserve(g.S);
     g.dfs_bound.clear();
-    int64_t visited = 0;
-    bool ok = true;
-    dfs_walk(g, [&](const int* lv) {
-        int64_t id = subspace_id(g, lv);
-        if (id < 0) { ok = false; return; }
-        g.dfs_base.push_back(g.sub_base[id]);
-        ++visited;
-    }, g.explicit_set ? &g.dfs_bound : nullptr);
-    if (!ok || visited != g.S) {
+    DfsWalk w{g, g.explicit_set ? explicit_tab_dims(g) : 0,
+              g.explicit_set ? &g.dfs_bound : nullptr, &g.dfs_base};
+    w.go(g.d - 1, g.n - 1);
+    if (!w.ok || int64_t(g.dfs_base.size()) != g.S) {
         set_error("build_launch_tables: depth-first walk does not cover the level set");
         return kInvalid;
     }

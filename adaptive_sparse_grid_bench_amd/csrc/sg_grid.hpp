@@ -27,6 +27,7 @@ namespace sg {
 constexpr int kMaxDims = 20;        // asg.cu:17 (MAX_NUM_DIMS)
 constexpr int kMaxLevel = 15;       // per-dimension level cap of the kernels' tables
 constexpr int kMaxLevelSum = 26;    // keeps 2^s * sizeof(double) inside 32-bit offsets
+constexpr int kExplicitTabDims = 3; // register-tabled dims of the explicit-set walk
 
 struct HierTables {
     // One entry per tile / family / block of the pole-family decomposition of one
@@ -77,6 +78,7 @@ const std::string& last_error();
 int build_regular(Grid& g, int d, int n, const int* limits);
 int build_explicit(Grid& g, int d, const uint8_t* levels, int64_t count);
 int build_launch_tables(Grid& g, int hier_tile_values);
+int explicit_tab_dims(const Grid& g);   // tabled dims the explicit-set node codes assume
 
 int64_t subspace_id(const Grid& g, const int* levels);   // -1 if absent
 int64_t l2idx(const Grid& g, const int* levels);        // rank within its level sum

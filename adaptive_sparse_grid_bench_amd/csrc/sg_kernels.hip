@@ -24,6 +24,8 @@ namespace {
 
 constexpr int kThreads = 256;
 
+
+
 #define SG_HIP_CHECK(expr)                                                        \
     do {                                                                          \
         hipError_t e_ = (expr);                                                   \
@@ -306,247 +308,8 @@ int hierarchize_impl(const Grid& g, const DeviceTables& t, T* v, hipStream_t s) 
 }
 
 // -------------------------------------------------------------------- evaluate --
-
-// Hat values and in-block byte offsets of one coordinate at levels 0..LT-1.
-// With m_l = x 2^(l+1) - (2 t_l + 1), t_l = floor(x 2^l):  hat_l = 1 - |m_l|,
-// t_{l+1} = 2 t_l + [m_l >= 0],  m_{l+1} = 2 m_l - (m_l >= 0 ? 1 : -1).  For x on the
-// 2^-24 lattice every step is exact, so hat_l equals asg.c:1257-1262's value.
-template <int LT, class T>
-__device__ __forceinline__ void level_table(T x, T (&ph)[LT], uint32_t (&ub)[LT]) {
-    T m = T(2) * x - T(1);
-    uint32_t t = 0;
-#pragma unroll
-    for (int l = 0; l < LT; ++l) {
-        ph[l] = T(1) - fabs(m);
-        ub[l] = t * uint32_t(sizeof(T));
-        const bool ge = m >= T(0);
-        t = 2u * t + (ge ? 1u : 0u);
-        m = T(2) * m + (ge ? T(-1) : T(1));
-    }
-}
-
-struct AccF64 {
-    double s = 0.0;
-    __device__ __forceinline__ void add(double w, double a) { s = fma(w, a, s); }
-    __device__ __forceinline__ double value() const { return s; }
-};
-
-// Kahan-compensated FP32 accumulation across subspaces
-struct AccKahan {
-    float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float w, float a) {
-        const float y = fmaf(w, a, -c);
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ float value() const { return s; }
-};
-
-template <class T>
-struct EvalParams {
-    const T* alpha;
-    const T* coords;            // [P][D] row-major
-    T* out;
-    int64_t P;
-    const int32_t* leaf_base;   // subspace base offsets in walk order
-    int n;                      // level sums 0..n-1
-    int limits[kMaxDims];
-};
-
-template <class T, class Acc>
-struct WalkState {
-    const unsigned char* alpha;
-    const int32_t* leaf_base;
-    int q;
-    Acc acc;
-    __device__ __forceinline__ void leaf(T w, uint32_t voff) {
-        const unsigned char* blk = alpha + size_t(leaf_base[q]) * sizeof(T);
-        ++q;
-        acc.add(w, *reinterpret_cast<const T*>(blk + voff));
-    }
-};
-
-// Per-lane state.  With LT <= 8 the hats and offsets of dims 0 and 1 at every level sit
-// in register tables (the two innermost loops are unrolled over them); with LT = 16 the
-// tables would not fit the unroller's budget and every dimension is walked incrementally.
-template <int D, int LT, class T>
-struct Lane {
-    static constexpr int kTab = LT <= 8 ? LT : 1;
-    T m0[D];                        // 2 x_k - 1 (m at level 0)
-    T ph0[kTab], ph1[kTab];         // hats of dims 0 and 1 at every level
-    uint32_t ub0[kTab], ub1[kTab];  // t_l * sizeof(T) of dims 0 and 1
-};
-
-// Depth-first walk: dimension K's level runs 0..min(rem, limit_K); dimension d-1 is
-// outermost, so for a fixed level sum the leaves come in rank order.  pp is the
-// product of the hats of dims > K, off the in-block byte offset they contribute,
-// hsum the sum of their levels (the shift of dimension K's index bits).
-template <int K, int D, int LT, class T, class Acc>
-struct Walker {
-    static __device__ __forceinline__ void run(const Lane<D, LT, T>& ln, const int* lim,
-                                               T pp, uint32_t off, int hsum, int rem,
-                                               WalkState<T, Acc>& st) {
-        const int b = min(rem, lim[K]);
-        T m = ln.m0[K];
-        uint32_t t = 0;
-        for (int l = 0; l <= b; ++l) {
-            const T phi = T(1) - fabs(m);
-            Walker<K - 1, D, LT, T, Acc>::run(ln, lim, pp * phi,
-                                              off + ((t * uint32_t(sizeof(T))) << hsum),
-                                              hsum + l, rem - l, st);
-            const bool ge = m >= T(0);
-            t = 2u * t + (ge ? 1u : 0u);
-            m = T(2) * m + (ge ? T(-1) : T(1));
-        }
-    }
-};
-
-// Dims 1 and 0 from the register tables, both loops unrolled (LT <= 8 keeps the
-// LT x LT bodies within the unroller's budget; LT = 16 walks dim 1 like the others).
-template <int D, int LT, class T, class Acc>
-struct Walker<1, D, LT, T, Acc> {
-    static __device__ __forceinline__ void run(const Lane<D, LT, T>& ln, const int* lim,
-                                               T pp, uint32_t off, int hsum, int rem,
-                                               WalkState<T, Acc>& st) {
-        if constexpr (LT > 8) {
-            const int b = min(rem, lim[1]);
-            T m = ln.m0[1];
-            uint32_t t = 0;
-            for (int l = 0; l <= b; ++l) {
-                const T phi = T(1) - fabs(m);
-                Walker<0, D, LT, T, Acc>::run(ln, lim, pp * phi,
-                                              off + ((t * uint32_t(sizeof(T))) << hsum),
-                                              hsum + l, rem - l, st);
-                const bool ge = m >= T(0);
-                t = 2u * t + (ge ? 1u : 0u);
-                m = T(2) * m + (ge ? T(-1) : T(1));
-            }
-        } else {
-            const int b1 = min(rem, lim[1]);
-#pragma unroll
-            for (int l1 = 0; l1 < LT; ++l1) {
-                if (l1 > b1) break;
-                const T pp1 = pp * ln.ph1[l1];
-                const uint32_t off1 = (ln.ub1[l1] << hsum) + off;
-                const int h1 = hsum + l1;
-                const int b0 = min(rem - l1, lim[0]);
-#pragma unroll
-                for (int l0 = 0; l0 < LT; ++l0) {
-                    if (l0 > b0) break;
-                    st.leaf(pp1 * ln.ph0[l0], (ln.ub0[l0] << h1) + off1);
-                }
-            }
-        }
-    }
-};
-
-template <int D, int LT, class T, class Acc>
-struct Walker<0, D, LT, T, Acc> {
-    static __device__ __forceinline__ void run(const Lane<D, LT, T>& ln, const int* lim,
-                                               T pp, uint32_t off, int hsum, int rem,
-                                               WalkState<T, Acc>& st) {
-        if constexpr (LT > 8) {                 // no tables: incremental hats as above
-            const int b = min(rem, lim[0]);
-            T m = ln.m0[0];
-            uint32_t t = 0;
-            for (int l = 0; l <= b; ++l) {
-                st.leaf(pp * (T(1) - fabs(m)), ((t * uint32_t(sizeof(T))) << hsum) + off);
-                const bool ge = m >= T(0);
-                t = 2u * t + (ge ? 1u : 0u);
-                m = T(2) * m + (ge ? T(-1) : T(1));
-            }
-        } else {
-            const int b0 = min(rem, lim[0]);
-#pragma unroll
-            for (int l0 = 0; l0 < LT; ++l0) {
-                if (l0 > b0) break;
-                st.leaf(pp * ln.ph0[l0], (ln.ub0[l0] << hsum) + off);
-            }
-        }
-    }
-};
-
-template <int D, int LT, class T, class Acc>
-__global__ __launch_bounds__(kThreads) void k_eval_dfs(EvalParams<T> prm) {
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool live = p < prm.P;
-    const int64_t pc = live ? p : 0;     // idle lanes walk point 0 and store nothing
-    Lane<D, LT, T> ln;
-    const T* x = prm.coords + pc * D;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const T xk = x[k];
-        if constexpr (LT <= 8) {
-            if (k == 0) level_table<LT, T>(xk, ln.ph0, ln.ub0);
-            else if (k == 1) level_table<LT, T>(xk, ln.ph1, ln.ub1);
-        }
-        ln.m0[k] = T(2) * xk - T(1);
-    }
-    WalkState<T, Acc> st{reinterpret_cast<const unsigned char*>(prm.alpha), prm.leaf_base, 0,
-                         Acc()};
-    Walker<D - 1, D, LT, T, Acc>::run(ln, prm.limits, T(1), 0u, 0, prm.n - 1, st);
-    if (live) prm.out[p] = st.acc.value();
-}
-
-template <int D, int LT, class T, class Acc>
-int launch_dfs(const EvalParams<T>& prm, int block, hipStream_t s) {
-    const int64_t blocks = (prm.P + block - 1) / block;
-    hipLaunchKernelGGL((k_eval_dfs<D, LT, T, Acc>), dim3(unsigned(blocks)), dim3(block), 0, s,
-                       prm);
-    return check_launch("k_eval_dfs");
-}
-
-template <int LT, class T, class Acc>
-int dispatch_dfs_d(int d, const EvalParams<T>& prm, int block, hipStream_t s) {
-    switch (d) {
-#define SG_CASE(DD) \
-    case DD: return launch_dfs<DD, LT, T, Acc>(prm, block, s);
-        SG_CASE(1) SG_CASE(2) SG_CASE(3) SG_CASE(4) SG_CASE(5) SG_CASE(6) SG_CASE(7)
-        SG_CASE(8) SG_CASE(9) SG_CASE(10) SG_CASE(11) SG_CASE(12) SG_CASE(13) SG_CASE(14)
-        SG_CASE(15) SG_CASE(16) SG_CASE(17) SG_CASE(18) SG_CASE(19) SG_CASE(20)
-#undef SG_CASE
-        default:
-            set_error("evaluate: unsupported number of dimensions");
-            return kUnsupported;
-    }
-}
-
-template <class T, class Acc>
-int evaluate_impl(const Grid& g, const DeviceTables& t, const T* alpha, const T* coords,
-                  int64_t P, T* out, const EvalTuning& tu, hipStream_t s) {
-    if (P == 0) return kOk;
-    if (g.explicit_set) {
-        set_error("evaluate: explicit level sets use the refined-subspace entry point");
-        return kUnsupported;
-    }
-    if (tu.block != 64 && tu.block != 128 && tu.block != 256) {
-        set_error("evaluate: block must be 64, 128 or 256");
-        return kInvalid;
-    }
-    if ((P + tu.block - 1) / tu.block > int64_t(0x7fffffff)) {
-        set_error("evaluate: too many points for one launch");
-        return kRange;
-    }
-    EvalParams<T> prm{};
-    prm.alpha = alpha;
-    prm.coords = coords;
-    prm.out = out;
-    prm.P = P;
-    prm.leaf_base = t.dfs_base;
-    prm.n = g.n;
-    int maxl = 0;
-    for (int j = 0; j < g.d; ++j) {
-        prm.limits[j] = g.limits[j];
-        maxl = std::max(maxl, g.limits[j]);
-    }
-    // dims 0 and 1 keep per-level tables of LT entries in registers
-    const int need = std::max(g.limits[0], g.d > 1 ? g.limits[1] : 0) + 1;
-    if (need <= 8) return dispatch_dfs_d<8, T, Acc>(g.d, prm, tu.block, s);
-    if (need <= 16) return dispatch_dfs_d<16, T, Acc>(g.d, prm, tu.block, s);
-    set_error("evaluate: level above 15 in dimension 0 or 1");
-    return kUnsupported;
-}
+// The depth-first walk kernels live in sg_eval.hpp and are instantiated in
+// sg_eval_f64.hip / sg_eval_f32.hip.
 
 // Reference operation order (asg.c:1250-1268): for each subspace in compact order the hat
 // product over k = 0..d-1 starting from 1, times the surplus, added to the running sum.
@@ -609,6 +372,95 @@ int evaluate_ref_impl(const Grid& g, const DeviceTables& t, const T* alpha, cons
             set_error("evaluate_ref: unsupported number of dimensions");
             return kUnsupported;
     }
+}
+
+// -------------------------------------------------------------- point binning --
+
+// key = interleaved leading bits of the coordinates: bit 1 of dims 0..d-1, then bit 2,
+// ...; the histogram of keys is accumulated in the same pass.
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_point_keys(const T* __restrict__ coords, int64_t P,
+                                                         int d, int key_bits,
+                                                         uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ hist) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < P;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const T* x = coords + i * d;
+        uint32_t key = 0;
+        const int levels = (key_bits + d - 1) / d;
+        for (int j = 0; j < levels; ++j) {
+            for (int k = 0; k < d; ++k) {
+                if (j * d + k >= key_bits) break;
+                const uint32_t t = static_cast<uint32_t>(ldexp(double(x[k]), j + 1));
+                key = (key << 1) | (t & 1u);
+            }
+        }
+        keys[i] = key;
+        atomicAdd(&hist[key], 1u);
+    }
+}
+
+// exclusive scan of n counts in place, one workgroup of kThreads threads
+__global__ __launch_bounds__(kThreads) void k_scan_counts(uint32_t* __restrict__ c, int64_t n) {
+    __shared__ uint32_t part[kThreads];
+    const int64_t chunk = (n + kThreads - 1) / kThreads;
+    const int64_t lo = threadIdx.x * chunk;
+    const int64_t hi = lo + chunk < n ? lo + chunk : n;
+    uint32_t sum = 0;
+    for (int64_t i = lo; i < hi; ++i) sum += c[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int i = 0; i < kThreads; ++i) {
+            const uint32_t v = part[i];
+            part[i] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (int64_t i = lo; i < hi; ++i) {
+        const uint32_t v = c[i];
+        c[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scatter_points(const uint32_t* __restrict__ keys,
+                                                             int64_t P,
+                                                             uint32_t* __restrict__ offs,
+                                                             uint32_t* __restrict__ perm) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < P;
+         i += int64_t(gridDim.x) * blockDim.x)
+        perm[atomicAdd(&offs[keys[i]], 1u)] = static_cast<uint32_t>(i);
+}
+
+__global__ __launch_bounds__(kThreads) void k_fill_u32(uint32_t* __restrict__ p, int64_t n,
+                                                       uint32_t v) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+template <class T>
+int sort_points_impl(const T* coords, int64_t P, int d, int key_bits, void* ws, uint32_t* perm,
+                     hipStream_t s) {
+    if (P == 0) return kOk;
+    if (P > int64_t(0xffffffff) || key_bits < 0 || key_bits > 24 || d < 1) {
+        set_error("sort_points: bad size");
+        return kInvalid;
+    }
+    uint32_t* keys = static_cast<uint32_t*>(ws);
+    uint32_t* hist = keys + P;
+    const int64_t nb = int64_t(1) << key_bits;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nb)), dim3(kThreads), 0, s, hist, nb, 0u);
+    hipLaunchKernelGGL(k_point_keys<T>, dim3(grid_for(P)), dim3(kThreads), 0, s, coords, P, d,
+                       key_bits, keys, hist);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kThreads), 0, s, hist, nb);
+    hipLaunchKernelGGL(k_scatter_points, dim3(grid_for(P)), dim3(kThreads), 0, s, keys, P, hist,
+                       perm);
+    return check_launch("sort_points");
 }
 
 // ------------------------------------------------------------------ utilities --
@@ -738,16 +590,6 @@ int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipSt
     return hierarchize_impl<float>(g, t, v, s);
 }
 
-int launch_evaluate_f64(const Grid& g, const DeviceTables& t, const double* alpha,
-                        const double* coords, int64_t P, double* out, const EvalTuning& tu,
-                        hipStream_t s) {
-    return evaluate_impl<double, AccF64>(g, t, alpha, coords, P, out, tu, s);
-}
-int launch_evaluate_f32c(const Grid& g, const DeviceTables& t, const float* alpha,
-                         const float* coords, int64_t P, float* out, const EvalTuning& tu,
-                         hipStream_t s) {
-    return evaluate_impl<float, AccKahan>(g, t, alpha, coords, P, out, tu, s);
-}
 int launch_evaluate_ref_f64(const Grid& g, const DeviceTables& t, const double* alpha,
                             const double* coords, int64_t P, double* out, hipStream_t s) {
     return evaluate_ref_impl<double>(g, t, alpha, coords, P, out, s);
@@ -755,6 +597,27 @@ int launch_evaluate_ref_f64(const Grid& g, const DeviceTables& t, const double* 
 int launch_evaluate_ref_f32(const Grid& g, const DeviceTables& t, const float* alpha,
                             const float* coords, int64_t P, float* out, hipStream_t s) {
     return evaluate_ref_impl<float>(g, t, alpha, coords, P, out, s);
+}
+
+int key_bits_for(int64_t P, int d) {
+    // about 32 points per key (a wave of 64 lanes spans two or three keys), at most 2^16
+    // keys so the one-workgroup scan stays short
+    int b = 0;
+    while (b < 16 && (int64_t(64) << b) <= P) ++b;
+    return std::min(b, 20 * d);
+}
+
+size_t sort_workspace_bytes(int64_t P, int key_bits) {
+    return size_t(P) * 4 + ((size_t(1) << key_bits) + 1) * 4;
+}
+
+int launch_sort_points_f64(const double* coords, int64_t P, int d, int key_bits, void* ws,
+                           uint32_t* perm, hipStream_t s) {
+    return sort_points_impl<double>(coords, P, d, key_bits, ws, perm, s);
+}
+int launch_sort_points_f32(const float* coords, int64_t P, int d, int key_bits, void* ws,
+                           uint32_t* perm, hipStream_t s) {
+    return sort_points_impl<float>(coords, P, d, key_bits, ws, perm, s);
 }
 
 int launch_uniform_points_f64(int64_t count, uint64_t seed, double* out, hipStream_t s) {

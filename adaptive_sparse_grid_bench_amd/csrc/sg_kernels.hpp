@@ -46,14 +46,28 @@ int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipSt
 
 struct EvalTuning {
     int block = 256;            // threads per workgroup
+    int variant = 0;            // register-table walk variant (sg_eval.hpp kVariants)
+    int bin_min_points = 4096;  // spatially bin the points of launches at least this big
 };
 
+// perm: lane order of the points (from sort_points) or null for the given order
 int launch_evaluate_f64(const Grid& g, const DeviceTables& t, const double* alpha,
                         const double* coords, int64_t P, double* out, const EvalTuning& tu,
-                        hipStream_t s);
+                        const uint32_t* perm, hipStream_t s);
 int launch_evaluate_f32c(const Grid& g, const DeviceTables& t, const float* alpha,
                          const float* coords, int64_t P, float* out, const EvalTuning& tu,
-                         hipStream_t s);
+                         const uint32_t* perm, hipStream_t s);
+
+// Spatial binning of query points: a key of k interleaved leading coordinate bits
+// (bit 1 of dims 0..d-1, then bit 2, ...), counting-sorted into perm[P].  Points that
+// share a key share their cell in every subspace whose levels stay within those bits.
+// Workspace: 4 (P + 2^k + 1) bytes, from sort_workspace_bytes.
+int key_bits_for(int64_t P, int d);
+size_t sort_workspace_bytes(int64_t P, int key_bits);
+int launch_sort_points_f64(const double* coords, int64_t P, int d, int key_bits, void* ws,
+                           uint32_t* perm, hipStream_t s);
+int launch_sort_points_f32(const float* coords, int64_t P, int d, int key_bits, void* ws,
+                           uint32_t* perm, hipStream_t s);
 int launch_evaluate_ref_f64(const Grid& g, const DeviceTables& t, const double* alpha,
                             const double* coords, int64_t P, double* out, hipStream_t s);
 int launch_evaluate_ref_f32(const Grid& g, const DeviceTables& t, const float* alpha,

@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsgrid.so")
+# SGRID_LIB overrides the library path (for comparing builds of the same sources)
+LIB_PATH = os.environ.get("SGRID_LIB") or os.path.join(_HERE, "libsgrid.so")
 
 SG_OK = 0
 SG_E_INVALID = -1
@@ -62,7 +63,7 @@ def _load() -> C.CDLL:
         "sg_grid_info": (i, [vp, ip, ip, llp, llp]),
         "sg_grid_tables": (i, [vp, llp, llp]),
         "sg_grid_subspaces": (i, [vp, C.POINTER(C.c_ubyte), llp]),
-        "sg_grid_set_tuning": (i, [vp, i, i]),
+        "sg_grid_set_tuning": (i, [vp, i, i, i, i]),
         "sg_gp2idx": (i, [vp, ip, ip, llp]),
         "sg_idx2gp": (i, [vp, ll, ip, ip]),
         "sg_idx2l": (i, [vp, ll, i, ip]),
@@ -172,8 +173,11 @@ class Grid:
                                     base.ctypes.data_as(C.POINTER(C.c_longlong))))
         return lv, base
 
-    def set_tuning(self, eval_block: int = 0, hier_tile_values: int = 0) -> None:
-        check(lib.sg_grid_set_tuning(self._h, eval_block, hier_tile_values))
+    def set_tuning(self, eval_block: int = 0, eval_variant: int = -1, bin_min_points: int = 0,
+                   hier_tile_values: int = 0) -> None:
+        """Launch tuning (see sg_grid_set_tuning); 0 / -1 keep the current value."""
+        check(lib.sg_grid_set_tuning(self._h, eval_block, eval_variant, bin_min_points,
+                                     hier_tile_values))
 
     # ---- host bijection (asg.c:187-279) --------------------------------------
     def gp2idx(self, levels, indices) -> int:

@@ -74,9 +74,16 @@ int sg_grid_tables(const sg_grid *grid, long long *dp, long long *lidx);
 /* level vectors [S][d] and base offsets [S] of the subspaces in compact order. */
 int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *base);
 
-/* Tuning of the launches: threads per workgroup of the evaluate kernels (64, 128 or
- * 256) and the target values per hierarchization tile (>= 64).  0 keeps a value. */
-int sg_grid_set_tuning(sg_grid *grid, int eval_block, int hier_tile_values);
+/* Tuning of the launches (not concurrently with other calls on the same grid):
+ *   eval_block        threads per workgroup of the evaluate kernels: 64, 128 or 256;
+ *   eval_variant      register-table walk variant 0..SG_EVAL_VARIANTS-1 (-1 keeps);
+ *   bin_min_points    evaluate launches with at least this many points first bin the
+ *                     points spatially (-1 never);
+ *   hier_tile_values  target values per hierarchization tile (>= 64).
+ * 0 keeps a value. */
+#define SG_EVAL_VARIANTS 3
+int sg_grid_set_tuning(sg_grid *grid, int eval_block, int eval_variant, int bin_min_points,
+                       int hier_tile_values);
 
 /* ------------------------------------------------------ host bijection ------- */
 

@@ -16,7 +16,8 @@
 struct sg_grid {
     sg::Grid g;
     sg::DeviceTables t;
-    sg::EvalTuning tune;
+    sg::EvalTuning tune;        // FP64 evaluate
+    sg::EvalTuning tune32;      // FP32 evaluate
     int hier_tile_values = 1024;
     std::mutex mu;
     bool uploaded = false;
@@ -52,6 +53,8 @@ int check_device(const sg_grid* hc) {
 }
 
 int finish_create(sg_grid* h, sg_grid** out) {
+    h->tune.variant = sg::default_variant(h->g.d, false);
+    h->tune32.variant = sg::default_variant(h->g.d, true);
     int st = sg::build_launch_tables(h->g, h->hier_tile_values);
     if (st != sg::kOk) {
         delete h;
@@ -70,7 +73,7 @@ int eval_one(const sg_grid* h, const double* a, const double* x, int64_t P, doub
 }
 int eval_one(const sg_grid* h, const float* a, const float* x, int64_t P, float* o,
              const uint32_t* perm, hipStream_t s) {
-    return sg::launch_evaluate_f32c(h->g, h->t, a, x, P, o, h->tune, perm, s);
+    return sg::launch_evaluate_f32c(h->g, h->t, a, x, P, o, h->tune32, perm, s);
 }
 int sort_one(const double* x, int64_t P, int d, int kb, void* ws, uint32_t* perm, hipStream_t s) {
     return sg::launch_sort_points_f64(x, P, d, kb, ws, perm, s);
@@ -173,22 +176,33 @@ int sg_grid_subspaces(const sg_grid* grid, unsigned char* levels, long long* bas
     return sg::kOk;
 }
 
-int sg_grid_set_tuning(sg_grid* grid, int eval_block, int eval_variant, int bin_min_points,
-                       int hier_tile_values) {
-    if (!grid) return fail(sg::kInvalid, "sg_grid_set_tuning: NULL grid");
-    if (eval_block != 0 && eval_block != 64 && eval_block != 128 && eval_block != 256)
+int sg_grid_get_tuning(const sg_grid* grid, sg_tuning* tu) {
+    if (!grid || !tu) return fail(sg::kInvalid, "sg_grid_get_tuning: NULL argument");
+    tu->eval_block = grid->tune.block;
+    tu->eval_variant_f64 = grid->tune.variant;
+    tu->eval_variant_f32 = grid->tune32.variant;
+    tu->bin_min_points = grid->tune.bin_min_points;
+    tu->hier_tile_values = grid->hier_tile_values;
+    return sg::kOk;
+}
+
+int sg_grid_set_tuning(sg_grid* grid, const sg_tuning* tu) {
+    if (!grid || !tu) return fail(sg::kInvalid, "sg_grid_set_tuning: NULL argument");
+    if (tu->eval_block != 64 && tu->eval_block != 128 && tu->eval_block != 256)
         return fail(sg::kInvalid, "sg_grid_set_tuning: eval_block must be 64, 128 or 256");
-    if (eval_variant < -1 || eval_variant >= SG_EVAL_VARIANTS)
-        return fail(sg::kInvalid, "sg_grid_set_tuning: unknown eval_variant");
-    if (hier_tile_values != 0 && hier_tile_values < 64)
-        return fail(sg::kInvalid, "sg_grid_set_tuning: hier_tile_values must be >= 64");
+    if (tu->eval_variant_f64 < 0 || tu->eval_variant_f64 >= SG_EVAL_VARIANTS ||
+        tu->eval_variant_f32 < 0 || tu->eval_variant_f32 >= SG_EVAL_VARIANTS)
+        return fail(sg::kInvalid, "sg_grid_set_tuning: unknown evaluate variant");
+    if (tu->hier_tile_values < 64 || tu->hier_tile_values > (1 << 24))
+        return fail(sg::kInvalid, "sg_grid_set_tuning: hier_tile_values out of range");
     std::lock_guard<std::mutex> lock(grid->mu);
-    if (eval_block) grid->tune.block = eval_block;
-    if (eval_variant >= 0) grid->tune.variant = eval_variant;
-    if (bin_min_points) grid->tune.bin_min_points = bin_min_points < 0 ? 0 : bin_min_points;
-    if (hier_tile_values && hier_tile_values != grid->hier_tile_values) {
-        grid->hier_tile_values = hier_tile_values;
-        int st = sg::build_launch_tables(grid->g, hier_tile_values);
+    grid->tune.block = grid->tune32.block = tu->eval_block;
+    grid->tune.variant = tu->eval_variant_f64;
+    grid->tune32.variant = tu->eval_variant_f32;
+    grid->tune.bin_min_points = grid->tune32.bin_min_points = tu->bin_min_points;
+    if (tu->hier_tile_values != grid->hier_tile_values) {
+        grid->hier_tile_values = tu->hier_tile_values;
+        int st = sg::build_launch_tables(grid->g, tu->hier_tile_values);
         if (st != sg::kOk) return st;
         if (grid->uploaded) {            // re-upload on the next device call
             sg::free_tables(grid->t);

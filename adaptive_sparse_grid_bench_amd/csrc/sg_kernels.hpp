@@ -50,6 +50,9 @@ struct EvalTuning {
     int bin_min_points = 4096;  // spatially bin the points of launches at least this big
 };
 
+// default variants by dimension and precision (measured at d = 10, 12, 16, 20)
+inline int default_variant(int d, bool f32) { return f32 ? 2 : (d >= 16 ? 1 : 0); }
+
 // perm: lane order of the points (from sort_points) or null for the given order
 int launch_evaluate_f64(const Grid& g, const DeviceTables& t, const double* alpha,
                         const double* coords, int64_t P, double* out, const EvalTuning& tu,

@@ -30,13 +30,26 @@ SG_E_UNSUPPORTED = -5
 EXPORTS = [
     "sg_last_error", "sg_status_string", "sg_version",
     "sg_grid_create", "sg_grid_create_explicit", "sg_grid_destroy", "sg_grid_info",
-    "sg_grid_tables", "sg_grid_subspaces", "sg_grid_set_tuning",
+    "sg_grid_tables", "sg_grid_subspaces", "sg_grid_get_tuning", "sg_grid_set_tuning",
     "sg_gp2idx", "sg_idx2gp", "sg_idx2l", "sg_l2idx",
     "sg_idx2gp_dev", "sg_gp2idx_dev", "sg_grid_coords_f64",
     "sg_hierarchize_f64", "sg_hierarchize_f32",
     "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
     "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
 ]
+
+
+class Tuning(C.Structure):
+    """sg_tuning (include/sgrid.h)."""
+    _fields_ = [("eval_block", C.c_int), ("eval_variant_f64", C.c_int),
+                ("eval_variant_f32", C.c_int), ("bin_min_points", C.c_int),
+                ("hier_tile_values", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EVAL_VARIANTS = 3
 
 
 class SgError(RuntimeError):
@@ -46,6 +59,13 @@ class SgError(RuntimeError):
 
 
 def _load() -> C.CDLL:
+    # torch ships its own HIP runtime under the same soname as /opt/rocm's; whichever
+    # is loaded first serves the whole process.  Load torch's first so that the tensors
+    # and streams it hands over belong to the runtime the library launches on.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                           "(make -C adaptive_sparse_grid_bench_amd/csrc)")
@@ -63,7 +83,8 @@ def _load() -> C.CDLL:
         "sg_grid_info": (i, [vp, ip, ip, llp, llp]),
         "sg_grid_tables": (i, [vp, llp, llp]),
         "sg_grid_subspaces": (i, [vp, C.POINTER(C.c_ubyte), llp]),
-        "sg_grid_set_tuning": (i, [vp, i, i, i, i]),
+        "sg_grid_get_tuning": (i, [vp, C.POINTER(Tuning)]),
+        "sg_grid_set_tuning": (i, [vp, C.POINTER(Tuning)]),
         "sg_gp2idx": (i, [vp, ip, ip, llp]),
         "sg_idx2gp": (i, [vp, ll, ip, ip]),
         "sg_idx2l": (i, [vp, ll, i, ip]),
@@ -173,11 +194,19 @@ class Grid:
                                     base.ctypes.data_as(C.POINTER(C.c_longlong))))
         return lv, base
 
-    def set_tuning(self, eval_block: int = 0, eval_variant: int = -1, bin_min_points: int = 0,
-                   hier_tile_values: int = 0) -> None:
-        """Launch tuning (see sg_grid_set_tuning); 0 / -1 keep the current value."""
-        check(lib.sg_grid_set_tuning(self._h, eval_block, eval_variant, bin_min_points,
-                                     hier_tile_values))
+    def get_tuning(self) -> dict:
+        tu = Tuning()
+        check(lib.sg_grid_get_tuning(self._h, C.byref(tu)))
+        return tu.as_dict()
+
+    def set_tuning(self, **kw) -> None:
+        """Update launch tuning fields (sg_tuning names); others keep their value."""
+        cur = self.get_tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise TypeError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(kw)
+        check(lib.sg_grid_set_tuning(self._h, C.byref(Tuning(**cur))))
 
     # ---- host bijection (asg.c:187-279) --------------------------------------
     def gp2idx(self, levels, indices) -> int:

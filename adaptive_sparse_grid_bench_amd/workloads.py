@@ -77,3 +77,76 @@ def f_gaussian_sum(x: np.ndarray, centers: np.ndarray, widths: np.ndarray) -> np
 def f_sin_prod(x: np.ndarray, c: float) -> np.ndarray:
     """prod_i sin(pi x_i) * (1 + 0.1 c), the batched-grid workload."""
     return np.prod(np.sin(np.pi * x), axis=1) * (1.0 + 0.1 * c)
+
+
+# --- refined-subspace (dimension-adaptive) level sets ---------------------------------
+
+def _surplus_1d_gauss(l: int, t: int, c: float, sigma: float) -> float:
+    """1-D hierarchical surplus of exp(-(x-c)^2 / (2 sigma^2)) at point (l, t)."""
+    g = lambda x: np.exp(-(x - c) ** 2 / (2.0 * sigma * sigma))
+    x = (2 * t + 1) / 2.0 ** (l + 1)
+    h = 2.0 ** -(l + 1)
+    left = g(x - h) if t > 0 else 0.0
+    right = g(x + h) if t < (1 << l) - 1 else 0.0
+    return g(x) - 0.5 * (left + right)
+
+
+def adaptive_peak(d: int, seed: int):
+    """Seeded peak centre, uniform in [0.3, 0.7]^d."""
+    u = uniform_u24(d, seed).astype(np.float64) * 2.0 ** -24
+    return 0.3 + 0.4 * u
+
+
+def adaptive_level_set(d: int = 8, base_n: int = 3, extra: int = 4, seed: int = 11,
+                       sigma: float = 0.15, max_subspaces: int = 2000, tol: float = 1e-4):
+    """Dimension-adaptive refinement of a regular base grid towards a Gaussian peak.
+
+    Starts from the regular set |l|_1 <= base_n - 1 and greedily adds admissible forward
+    neighbours l + e_k (every backward neighbour present, l_k + 1 <= base_n - 1 + extra)
+    in order of the surplus magnitude of f = exp(-|x - c|^2 / (2 sigma^2)) at the cell of
+    subspace l + e_k containing the peak c (the largest surplus of that subspace, a
+    product of 1-D surpluses), until max_subspaces or the indicator drops below tol.
+    Returns (levels [S][d] uint8, centre c).  The set is downward closed by construction.
+    """
+    import heapq
+    c = adaptive_peak(d, seed)
+    cap = base_n - 1 + extra
+    cache = {}
+
+    def indicator(l):
+        v = 1.0
+        for k, lk in enumerate(l):
+            key = (k, lk)
+            if key not in cache:
+                t = min(int(np.floor(c[k] * 2 ** lk)), (1 << lk) - 1)
+                cache[key] = abs(_surplus_1d_gauss(lk, t, c[k], sigma))
+            v *= cache[key]
+        return v
+
+    from itertools import product
+    current = {l for l in product(range(base_n), repeat=d) if sum(l) <= base_n - 1}
+    heap, seen = [], set()
+
+    def push_neighbours(l):
+        for k in range(d):
+            if l[k] + 1 > cap:
+                continue
+            m = list(l)
+            m[k] += 1
+            m = tuple(m)
+            if m in current or m in seen:
+                continue
+            if all(m[j] == 0 or tuple(m[i] - (i == j) for i in range(d)) in current
+                   for j in range(d)):
+                seen.add(m)
+                heapq.heappush(heap, (-indicator(m), m))
+
+    for l in sorted(current):
+        push_neighbours(l)
+    while heap and len(current) < max_subspaces:
+        neg, m = heapq.heappop(heap)
+        if -neg < tol:
+            break
+        current.add(m)
+        push_neighbours(m)      # every newly admissible set member has m as a parent
+    return np.array(sorted(current), np.uint8), c

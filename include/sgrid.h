@@ -74,16 +74,27 @@ int sg_grid_tables(const sg_grid *grid, long long *dp, long long *lidx);
 /* level vectors [S][d] and base offsets [S] of the subspaces in compact order. */
 int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *base);
 
-/* Tuning of the launches (not concurrently with other calls on the same grid):
+/* Launch tuning of one grid.  sg_grid_get_tuning returns the current values (defaults
+ * chosen from the grid's shape); sg_grid_set_tuning replaces them (not concurrently
+ * with other calls on the same grid).  The autotuner (package module autotune.py)
+ * times the candidates on the GPU and stores its choice per grid shape.
  *   eval_block        threads per workgroup of the evaluate kernels: 64, 128 or 256;
- *   eval_variant      register-table walk variant 0..SG_EVAL_VARIANTS-1 (-1 keeps);
- *   bin_min_points    evaluate launches with at least this many points first bin the
- *                     points spatially (-1 never);
- *   hier_tile_values  target values per hierarchization tile (>= 64).
- * 0 keeps a value. */
+ *   eval_variant_f64  register-table walk variant of the FP64 evaluate,
+ *   eval_variant_f32  and of the FP32 one: 0..SG_EVAL_VARIANTS-1 (tabled dimensions,
+ *                     gathers per batch and waves per SIMD; see the design notes);
+ *   bin_min_points    evaluate launches with at least this many points bin the points
+ *                     spatially first (<= 0: never);
+ *   hier_tile_values  target values per hierarchization workgroup tile (>= 64). */
 #define SG_EVAL_VARIANTS 3
-int sg_grid_set_tuning(sg_grid *grid, int eval_block, int eval_variant, int bin_min_points,
-                       int hier_tile_values);
+typedef struct sg_tuning {
+    int eval_block;
+    int eval_variant_f64;
+    int eval_variant_f32;
+    int bin_min_points;
+    int hier_tile_values;
+} sg_tuning;
+int sg_grid_get_tuning(const sg_grid *grid, sg_tuning *tuning);
+int sg_grid_set_tuning(sg_grid *grid, const sg_tuning *tuning);
 
 /* ------------------------------------------------------ host bijection ------- */
 

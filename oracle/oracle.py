@@ -67,6 +67,18 @@ def _load():
         fn = getattr(lib, f"sgo_evaluate_brute_{sfx}")
         fn.argtypes = [C.c_int, C.c_int, _i64p, _i64p, C.c_int64, fp, fp, C.c_int64, fp]
         fn.restype = None
+    _u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+    lib.sgo_x_offsets.argtypes = [C.c_int, C.c_int64, _u8p, _i64p]
+    lib.sgo_x_offsets.restype = C.c_int64
+    lib.sgo_x_find.argtypes = [C.c_int, C.c_int64, _u8p, _u8p]
+    lib.sgo_x_find.restype = C.c_int64
+    for sfx, fp in (("f64", _f64p), ("f32", _f32p)):
+        fn = getattr(lib, f"sgo_x_hierarchize_{sfx}")
+        fn.argtypes = [C.c_int, C.c_int64, _u8p, _i64p, fp]
+        fn.restype = None
+        fn = getattr(lib, f"sgo_x_evaluate_{sfx}")
+        fn.argtypes = [C.c_int, C.c_int64, _u8p, _i64p, fp, fp, C.c_int64, fp]
+        fn.restype = None
     _lib = lib
     return lib
 
@@ -176,3 +188,75 @@ def grid_coords(sh: Shape, dtype=np.float64) -> np.ndarray:
     """Coordinates (2 i + 1) / 2^(l+1) of every grid point in compact order (asg.c:150)."""
     lv, iv = idx2gp_all(sh)
     return ((2 * iv.astype(np.float64) + 1) / np.exp2(lv.astype(np.float64) + 1)).astype(dtype)
+
+
+# ------------------------------------------------ explicit (refined) level sets --
+
+@dataclass
+class XShape:
+    """An explicit downward-closed level set in compact order with its offsets."""
+    d: int
+    levels: np.ndarray      # [S][d] uint8, compact order
+    base: np.ndarray        # [S] int64
+    N: int
+    S: int
+
+
+def compact_sort(levels) -> np.ndarray:
+    """Sort level vectors into compact order: level sum, then l_{d-1}, l_{d-2}, ..."""
+    lv = np.unique(np.ascontiguousarray(levels, np.uint8), axis=0)
+    keys = [lv[:, j] for j in range(lv.shape[1])] + [lv.sum(1)]
+    return np.ascontiguousarray(lv[np.lexsort(keys)])
+
+
+def xshape(levels) -> XShape:
+    lv = compact_sort(levels)
+    S, d = lv.shape
+    base = np.zeros(S, np.int64)
+    N = _load().sgo_x_offsets(d, S, lv, base)
+    if N < 0:
+        raise ValueError("level list not in compact order")
+    return XShape(d, lv, base, int(N), S)
+
+
+def x_grid_coords(xs: XShape, dtype=np.float64) -> np.ndarray:
+    """Grid-point coordinates of an explicit set in compact order."""
+    out = np.zeros((xs.N, xs.d))
+    for q in range(xs.S):
+        l = xs.levels[q].astype(np.int64)
+        npts = 1 << int(l.sum())
+        r = np.arange(npts)
+        for k in range(xs.d - 1, -1, -1):
+            i = r & ((1 << int(l[k])) - 1)
+            r = r >> int(l[k])
+            out[xs.base[q]:xs.base[q] + npts, k] = (2 * i + 1) / 2.0 ** (l[k] + 1)
+    return out.astype(dtype)
+
+
+def x_hierarchize(xs: XShape, values: np.ndarray) -> np.ndarray:
+    v = np.array(values, copy=True, order="C")
+    getattr(_load(), f"sgo_x_hierarchize_{_sfx(v.dtype)}")(xs.d, xs.S, xs.levels, xs.base, v)
+    return v
+
+
+def x_evaluate(xs: XShape, alpha: np.ndarray, coords: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(alpha)
+    x = np.ascontiguousarray(coords, a.dtype)
+    out = np.zeros(x.shape[0], a.dtype)
+    getattr(_load(), f"sgo_x_evaluate_{_sfx(a.dtype)}")(xs.d, xs.S, xs.levels, xs.base, a, x,
+                                                        x.shape[0], out)
+    return out
+
+
+def x_evaluate_brute(xs: XShape, alpha: np.ndarray, coords: np.ndarray) -> np.ndarray:
+    """Sum over every grid point of alpha * prod of hats (numpy; tiny sets only)."""
+    gx = x_grid_coords(xs)
+    lv = np.zeros((xs.N, xs.d))
+    for q in range(xs.S):
+        npts = 1 << int(xs.levels[q].astype(np.int64).sum())
+        lv[xs.base[q]:xs.base[q] + npts] = xs.levels[q]
+    out = np.zeros(len(coords))
+    for p, x in enumerate(np.asarray(coords, np.float64)):
+        h = np.clip(1.0 - np.abs(x[None, :] * 2.0 ** (lv + 1) - gx * 2.0 ** (lv + 1)), 0, None)
+        out[p] = (np.prod(h, axis=1) * alpha).sum()
+    return out

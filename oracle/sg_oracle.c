@@ -182,3 +182,59 @@ int sgo_right_parent(int l, int i, int *pl, int *pi)
 #include "sg_oracle_fp.h"
 #undef REAL
 #undef SFX
+
+/* --------------------------------------------- explicit (refined) level sets --- */
+/* The same operations over an explicit downward-closed set of level vectors given in
+ * compact order (level sum ascending, then l_{d-1}, l_{d-2}, ... ascending), the
+ * generalisation of the restriction vector (asg.c:82-83, 113-129).  Offsets follow the
+ * same compact rule: subspace blocks of 2^|l| values in list order. */
+
+static int sgo_x_cmp(const unsigned char *a, const unsigned char *b, int d)
+{
+    int sa = 0, sb = 0;
+    for (int j = 0; j < d; ++j) { sa += a[j]; sb += b[j]; }
+    if (sa != sb) return sa < sb ? -1 : 1;
+    for (int j = d - 1; j >= 0; --j)
+        if (a[j] != b[j]) return a[j] < b[j] ? -1 : 1;
+    return 0;
+}
+
+/* index of level vector l in the sorted list, or -1 */
+i64 sgo_x_find(int d, i64 S, const unsigned char *levels, const unsigned char *l)
+{
+    i64 lo = 0, hi = S;
+    while (lo < hi) {
+        i64 mid = (lo + hi) / 2;
+        int c = sgo_x_cmp(levels + mid * d, l, d);
+        if (c < 0) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < S && sgo_x_cmp(levels + lo * d, l, d) == 0) return lo;
+    return -1;
+}
+
+/* base offsets of the subspaces and N; returns -1 if the list is not sorted */
+i64 sgo_x_offsets(int d, i64 S, const unsigned char *levels, i64 *base)
+{
+    i64 off = 0;
+    for (i64 q = 0; q < S; ++q) {
+        if (q > 0 && sgo_x_cmp(levels + (q - 1) * d, levels + q * d, d) >= 0) return -1;
+        int s = 0;
+        for (int j = 0; j < d; ++j) s += levels[q * d + j];
+        base[q] = off;
+        off += ((i64)1) << s;
+    }
+    return off;
+}
+
+#define REAL double
+#define SFX(name) name##_f64
+#include "sg_oracle_x.h"
+#undef REAL
+#undef SFX
+
+#define REAL float
+#define SFX(name) name##_f32
+#include "sg_oracle_x.h"
+#undef REAL
+#undef SFX

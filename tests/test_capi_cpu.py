@@ -65,6 +65,12 @@ def test_host_errors():
         g.idx2l(0, 4)                        # level sum >= n
     with pytest.raises(sgrid.SgError):
         g.set_tuning(eval_block=96)
+    with pytest.raises(sgrid.SgError):
+        g.set_tuning(eval_variant_f64=sgrid.EVAL_VARIANTS)
+    with pytest.raises(TypeError):
+        g.set_tuning(points_per_lane=2)
+    g.set_tuning(eval_block=128, bin_min_points=-1)
+    assert g.get_tuning()["eval_block"] == 128 and g.get_tuning()["bin_min_points"] == -1
 
 
 def test_explicit_set_matches_restricted_regular():

@@ -270,3 +270,80 @@ def test_evaluate_f32c_error_bound(sg, torch_gpu):
     err = np.abs(host(out).astype(np.float64) - ref)
     print("f32c max err", err.max(), "rms", np.sqrt((err ** 2).mean()))
     assert err.max() <= 1e-6
+
+
+# ------------------------------------------------ variants, binning, explicit sets --
+
+@pytest.mark.parametrize("variant", range(3))
+@pytest.mark.parametrize("binning", [-1, 1])
+@pytest.mark.parametrize("d,n,P", [(10, 8, 3000), (5, 6, 5000), (2, 8, 4000)])
+def test_evaluate_variants_and_binning(sg, torch_gpu, variant, binning, d, n, P):
+    """Every compiled walk variant, with and without spatial binning, in FP64 and FP32."""
+    torch = torch_gpu
+    sh = O.tables(d, n)
+    alpha = O.hierarchize(sh, sample(sh, "mixed"))
+    pts = W.uniform_points(P, d, seed=31 + d)
+    ref = O.evaluate(sh, alpha, pts)
+    absref = O.evaluate(sh, np.abs(alpha), pts)
+    g = sg.Grid(d, n)
+    g.set_tuning(eval_variant_f64=variant, eval_variant_f32=variant, bin_min_points=binning)
+    out = torch.full((P,), np.nan, dtype=torch.float64, device="cuda")
+    g.evaluate(dev(torch, alpha), dev(torch, pts), out)
+    check_f64(host(out), ref, absref)
+    o32 = torch.full((P,), np.nan, dtype=torch.float32, device="cuda")
+    g.evaluate(dev(torch, alpha.astype(np.float32)), dev(torch, pts.astype(np.float32)), o32)
+    err = np.abs(host(o32).astype(np.float64) - ref)
+    assert (err <= 2e-6 * absref + 1e-7).all(), err.max()
+
+
+def explicit_sets():
+    out = []
+    lv = np.array([O.idx2l(O.tables(4, 5, [4, 1, 4, 2]), r, s) for s in range(5)
+                   for r in range(int(O.tables(4, 5, [4, 1, 4, 2]).dp.reshape(4, 5)[3, s]))],
+                  np.uint8)
+    out.append(("regular_as_explicit", lv))
+    out.append(("adaptive_d8", W.adaptive_level_set(d=8, base_n=3, extra=4, seed=11)[0]))
+    out.append(("adaptive_d3", W.adaptive_level_set(d=3, base_n=3, extra=4, seed=3,
+                                                     max_subspaces=60)[0]))
+    # dimension 0 refined past level 7: the table-free walk
+    deep = [(a, b) for a in range(11) for b in range(3) if a + b <= 10]
+    out.append(("deep_dim0", np.array(deep, np.uint8)))
+    return out
+
+
+@pytest.mark.parametrize("name,levels", explicit_sets(), ids=[n for n, _ in explicit_sets()])
+def test_explicit_sets(sg, torch_gpu, name, levels):
+    torch = torch_gpu
+    xs = O.xshape(levels)
+    g = sg.Grid.explicit(xs.d, levels)
+    assert (g.N, g.S) == (xs.N, xs.S)
+    glv, gbase = g.subspaces()
+    assert (glv == xs.levels).all() and (gbase == xs.base).all()
+    x = O.x_grid_coords(xs)
+    # device bijection and coordinates
+    c = torch.empty((g.N, xs.d), dtype=torch.float64, device="cuda")
+    g.grid_coords(c)
+    assert (host(c) == x).all()
+    lv = torch.empty((g.N, xs.d), dtype=torch.int32, device="cuda")
+    iv = torch.empty_like(lv)
+    g.idx2gp_dev(lv, iv)
+    idx = torch.empty(g.N, dtype=torch.int32, device="cuda")
+    g.gp2idx_dev(lv, iv, idx, g.N)
+    assert (host(idx) == np.arange(g.N)).all()
+    # hierarchize bit-exact, evaluate within the FP64 bound
+    v = W.f_gauss(x) - 0.5 * W.f_reference(x)
+    t = dev(torch, v)
+    g.hierarchize(t)
+    alpha = O.x_hierarchize(xs, v)
+    assert (host(t) == alpha).all()
+    pts = W.uniform_points(3000, xs.d, seed=17)
+    ref = O.x_evaluate(xs, alpha, pts)
+    absref = O.x_evaluate(xs, np.abs(alpha), pts)
+    out = torch.empty(3000, dtype=torch.float64, device="cuda")
+    for binning in (-1, 1):
+        g.set_tuning(bin_min_points=binning)
+        g.evaluate(t, dev(torch, pts), out)
+        check_f64(host(out), ref, absref)
+    o32 = torch.empty(3000, dtype=torch.float32, device="cuda")
+    g.evaluate(dev(torch, alpha.astype(np.float32)), dev(torch, pts.astype(np.float32)), o32)
+    assert (np.abs(host(o32) - ref) <= 2e-6 * absref + 1e-7).all()

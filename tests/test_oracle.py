@@ -328,3 +328,59 @@ def test_uniform_points_are_lattice_and_seeded():
     assert (k == np.floor(k)).all() and (a >= 0).all() and (a < 1).all()
     assert (a.astype(np.float32).astype(np.float64) == a).all()
     assert abs(a.mean() - 0.5) < 0.02
+
+
+# ------------------------------------------------------ explicit (refined) sets --
+
+def regular_levels(d, n, lim=None):
+    sh = O.tables(d, n, lim)
+    return np.array([O.idx2l(sh, r, s) for s in range(n)
+                     for r in range(int(sh.dp.reshape(d, n)[d - 1, s]))], np.uint8)
+
+
+@pytest.mark.parametrize("d,n,lim", [(3, 5, None), (4, 5, [4, 1, 4, 2])])
+def test_explicit_oracle_equals_restricted_regular(d, n, lim):
+    """The explicit-set restatement on a restricted-regular set is bit-identical to the
+    dp-based oracle (itself pinned to the reference's golden vectors)."""
+    sh = O.tables(d, n, lim)
+    lv = regular_levels(d, n, lim)
+    rng = np.random.default_rng(0)
+    xs = O.xshape(lv[rng.permutation(len(lv))])
+    assert xs.N == sh.N and (xs.levels == lv).all()
+    assert (O.x_grid_coords(xs) == O.grid_coords(sh)).all()
+    v = W.f_gauss(O.grid_coords(sh)) - W.f_reference(O.grid_coords(sh))
+    a = O.hierarchize(sh, v)
+    assert (O.x_hierarchize(xs, v) == a).all()
+    pts = W.uniform_points(500, d, seed=2)
+    assert (O.x_evaluate(xs, a, pts) == O.evaluate(sh, a, pts)).all()
+
+
+def test_adaptive_level_set_properties():
+    lv, c = W.adaptive_level_set(d=8, base_n=3, extra=4, seed=11)
+    lv2, _ = W.adaptive_level_set(d=8, base_n=3, extra=4, seed=11)
+    assert (lv == lv2).all()
+    assert lv.max() <= 2 + 4 and (0.3 <= c).all() and (c <= 0.7).all()
+    members = set(map(tuple, lv))
+    for l in members:
+        for j in range(8):
+            if l[j]:
+                assert tuple(l[i] - (i == j) for i in range(8)) in members
+    # the regular base grid is contained
+    assert all(tuple(l) in members for l in regular_levels(8, 3))
+
+
+def test_explicit_adaptive_parabola_closed_form_and_brute_force():
+    lv, _ = W.adaptive_level_set(d=3, base_n=2, extra=3, seed=5, max_subspaces=30)
+    xs = O.xshape(lv)
+    x = O.x_grid_coords(xs)
+    lsum = np.zeros(xs.N)
+    for q in range(xs.S):
+        npts = 1 << int(xs.levels[q].astype(np.int64).sum())
+        lsum[xs.base[q]:xs.base[q] + npts] = xs.levels[q].astype(np.int64).sum()
+    a = O.x_hierarchize(xs, W.f_prod_parabola(x))
+    assert (a == 4.0 ** -lsum).all()
+    g = O.x_hierarchize(xs, W.f_gauss(x))
+    pts = W.uniform_points(200, 3, seed=8)
+    assert np.allclose(O.x_evaluate(xs, g, pts), O.x_evaluate_brute(xs, g, pts),
+                       rtol=1e-14, atol=1e-15)
+    assert np.allclose(O.x_evaluate(xs, g, x), W.f_gauss(x), rtol=0, atol=1e-15)

@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/sgrid.h"
+#include "sg_batch.hpp"
 #include "sg_grid.hpp"
 #include "sg_kernels.hpp"
 
@@ -104,6 +105,22 @@ int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, v
     return st;
 }
 
+}  // namespace
+
+struct sg_batch {
+    sg::Batch b;
+    int device = -1;
+    bool points_set = false;
+};
+
+namespace {
+int batch_device(const sg_batch* b) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return fail(sg::kHip, "hipGetDevice failed");
+    if (cur != b->device) return fail(sg::kInvalid, "current device differs from the batch's");
+    if (!b->points_set) return fail(sg::kInvalid, "sg_batch_set_points has not been called");
+    return sg::kOk;
+}
 }  // namespace
 
 extern "C" {
@@ -333,6 +350,117 @@ int sg_convert_f64_to_f32(const double* d_in, float* d_out, long long count, voi
     if (count < 0 || (count > 0 && (!d_in || !d_out)))
         return fail(sg::kInvalid, "sg_convert_f64_to_f32: bad argument");
     return sg::launch_f64_to_f32(d_in, d_out, count, as_stream(stream));
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int sg_batch_create(sg_grid* const* grids, int count, sg_batch** out) {
+    if (!grids || count < 1 || !out) return fail(sg::kInvalid, "sg_batch_create: bad argument");
+    *out = nullptr;
+    std::vector<sg::Batch::GridRef> refs;
+    for (int i = 0; i < count; ++i) {
+        if (!grids[i]) return fail(sg::kInvalid, "sg_batch_create: NULL grid");
+        int st = check_device(grids[i]);           // uploads the grid's tables
+        if (st != sg::kOk) return st;
+        refs.push_back({&grids[i]->g, &grids[i]->t});
+    }
+    sg_batch* b = new (std::nothrow) sg_batch();
+    if (!b) return fail(sg::kNoMem, "sg_batch_create: host allocation failed");
+    (void)hipGetDevice(&b->device);
+    int st = b->b.build(std::move(refs));
+    if (st != sg::kOk) {
+        b->b.release();
+        delete b;
+        return st;
+    }
+    *out = b;
+    return sg::kOk;
+}
+
+int sg_batch_destroy(sg_batch* batch) {
+    if (!batch) return sg::kOk;
+    batch->b.release();
+    delete batch;
+    return sg::kOk;
+}
+
+int sg_batch_values(const sg_batch* batch, long long* total, long long* value_offsets) {
+    if (!batch) return fail(sg::kInvalid, "sg_batch_values: NULL batch");
+    if (total) *total = batch->b.value_off.back();
+    if (value_offsets)
+        for (size_t i = 0; i < batch->b.value_off.size(); ++i) value_offsets[i] = batch->b.value_off[i];
+    return sg::kOk;
+}
+
+int sg_batch_set_points(sg_batch* batch, const long long* counts, long long* point_offsets,
+                        long long* coord_offsets) {
+    if (!batch || !counts) return fail(sg::kInvalid, "sg_batch_set_points: NULL argument");
+    std::lock_guard<std::mutex> lock(batch->b.mu);
+    std::vector<int64_t> c(counts, counts + batch->b.grids.size());
+    int st = batch->b.set_points(c);
+    if (st != sg::kOk) return st;
+    batch->points_set = true;
+    for (size_t i = 0; i < batch->b.point_off.size(); ++i) {
+        if (point_offsets) point_offsets[i] = batch->b.point_off[i];
+        if (coord_offsets) coord_offsets[i] = batch->b.coord_off[i];
+    }
+    return sg::kOk;
+}
+
+int sg_batch_hierarchize_f64(sg_batch* batch, double* d_values, void* stream) {
+    if (!batch || !d_values) return fail(sg::kInvalid, "sg_batch_hierarchize_f64: NULL argument");
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    if (cur != batch->device) return fail(sg::kInvalid, "current device differs from the batch's");
+    return batch->b.hierarchize_f64(d_values, as_stream(stream));
+}
+
+int sg_batch_hierarchize_f32(sg_batch* batch, float* d_values, void* stream) {
+    if (!batch || !d_values) return fail(sg::kInvalid, "sg_batch_hierarchize_f32: NULL argument");
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    if (cur != batch->device) return fail(sg::kInvalid, "current device differs from the batch's");
+    return batch->b.hierarchize_f32(d_values, as_stream(stream));
+}
+
+int sg_batch_evaluate_f64(sg_batch* batch, const double* d_alpha, const double* d_coords,
+                          double* d_out, void* stream) {
+    if (!batch || !d_alpha || !d_coords || !d_out)
+        return fail(sg::kInvalid, "sg_batch_evaluate_f64: NULL argument");
+    int st = batch_device(batch);
+    if (st != sg::kOk) return st;
+    return batch->b.evaluate_f64(d_alpha, d_coords, d_out, as_stream(stream));
+}
+
+int sg_batch_evaluate_f32c(sg_batch* batch, const float* d_alpha, const float* d_coords,
+                           float* d_out, void* stream) {
+    if (!batch || !d_alpha || !d_coords || !d_out)
+        return fail(sg::kInvalid, "sg_batch_evaluate_f32c: NULL argument");
+    int st = batch_device(batch);
+    if (st != sg::kOk) return st;
+    return batch->b.evaluate_f32c(d_alpha, d_coords, d_out, as_stream(stream));
+}
+
+int sg_batch_run_graph_f64(sg_batch* batch, double* d_alpha, const double* d_coords,
+                           double* d_out, int with_hierarchize, void* stream) {
+    if (!batch || !d_alpha || !d_coords || !d_out)
+        return fail(sg::kInvalid, "sg_batch_run_graph_f64: NULL argument");
+    int st = batch_device(batch);
+    if (st != sg::kOk) return st;
+    return batch->b.run_graph_f64(d_alpha, d_coords, d_out, with_hierarchize != 0,
+                                  as_stream(stream));
+}
+
+int sg_batch_run_graph_f32c(sg_batch* batch, float* d_alpha, const float* d_coords,
+                            float* d_out, int with_hierarchize, void* stream) {
+    if (!batch || !d_alpha || !d_coords || !d_out)
+        return fail(sg::kInvalid, "sg_batch_run_graph_f32c: NULL argument");
+    int st = batch_device(batch);
+    if (st != sg::kOk) return st;
+    return batch->b.run_graph_f32c(d_alpha, d_coords, d_out, with_hierarchize != 0,
+                                   as_stream(stream));
 }
 
 }  // extern "C"

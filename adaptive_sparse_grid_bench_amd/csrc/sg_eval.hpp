@@ -21,8 +21,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 #include "sg_grid.hpp"
+#include "sg_kernels.hpp"
 
 namespace sg {
 namespace eval {
@@ -32,15 +34,12 @@ constexpr int kThreads = 256;
 template <class T>
 struct Params {
     const T* alpha;
-    const T* coords;            // [P][D] row-major
+    const T* coords;            // row-major [P][D] per grid
     T* out;
-    int64_t P;
-    const int32_t* leaf_base;   // subspace base offsets in walk order
-    const uint8_t* bounds;      // explicit sets: one code per walk node
-    const uint32_t* perm;       // lane order of the points (spatially binned) or null
-    uint32_t alpha_bytes;       // N * sizeof(T): the buffer range of the surpluses
-    int n;                      // level sums 0..n-1
-    int limits[kMaxDims];
+    const GridDesc* grids;
+    const WorkItem* work;       // null: one grid (grids[0]), workgroups tile [0, P)
+    const uint32_t* perm;       // one-grid launches: lane order of the points, or null
+    int64_t P;                  // one-grid launches: number of points
 };
 
 struct AccF64 {
@@ -276,13 +275,27 @@ struct Walker {
 template <int D, int LT, int J, int B, int W, bool EXPL, class T, class Acc>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
 void k_eval(Params<T> prm) {
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool live = p < prm.P;
-    // idle lanes walk point 0 and store nothing; with a permutation, neighbouring lanes
-    // hold spatially close points, so a wave's gathers share cache lines
-    const int64_t pc = live ? (prm.perm ? int64_t(prm.perm[p]) : p) : 0;
+    int gi;
+    int64_t first, count;
+    if (prm.work) {
+        const WorkItem w = prm.work[blockIdx.x];
+        gi = w.grid;
+        first = w.first;
+        count = w.count;
+    } else {
+        gi = 0;
+        first = int64_t(blockIdx.x) * blockDim.x;
+        count = prm.P - first < int64_t(blockDim.x) ? prm.P - first : int64_t(blockDim.x);
+    }
+    const GridDesc& G = prm.grids[gi];
+    const int64_t p = first + threadIdx.x;
+    const bool live = int64_t(threadIdx.x) < count;
+    // idle lanes walk the workgroup's first point and store nothing; with a
+    // permutation, neighbouring lanes hold spatially close points, so a wave's gathers
+    // share cache lines
+    const int64_t pc = live ? (prm.perm ? int64_t(prm.perm[p]) : p) : first;
     Lane<D, LT, J, T> ln;
-    const T* x = prm.coords + pc * D;
+    const T* x = prm.coords + G.coord_off + pc * D;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         const T xk = x[k];
@@ -291,12 +304,16 @@ void k_eval(Params<T> prm) {
         }
         ln.m0[k] = T(2) * xk - T(1);
     }
+    int lim[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) lim[k] = G.limits[k];
     // buffer descriptor word 3 for gfx9-family raw (untyped) buffer access
-    State<T, Acc> st{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(prm.alpha), short(0),
-                                                       int(prm.alpha_bytes), 0x00020000),
-                     prm.leaf_base, prm.bounds, 0, 0, Acc()};
-    Walker<D - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, prm.limits, T(1), 0u, 0, prm.n - 1, st);
-    if (live) prm.out[pc] = st.acc.value();
+    State<T, Acc> st{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(prm.alpha + G.alpha_off),
+                                                       short(0), int(G.N * int64_t(sizeof(T))),
+                                                       0x00020000),
+                     G.leaf_base, G.bounds, 0, 0, Acc()};
+    Walker<D - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, lim, T(1), 0u, 0, G.n - 1, st);
+    if (live) prm.out[G.out_off + pc] = st.acc.value();
 }
 
 // Compiled variants of the register-table walk (LT = 8); the autotuner picks one per
@@ -311,8 +328,7 @@ constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 constexpr int kExplicitJ = kExplicitTabDims;
 
 template <int D, int LT, int J, int B, int W, bool EXPL, class T, class Acc>
-hipError_t launch_one(const Params<T>& prm, int block, hipStream_t s) {
-    const int64_t blocks = (prm.P + block - 1) / block;
+hipError_t launch_one(const Params<T>& prm, int64_t blocks, int block, hipStream_t s) {
     constexpr int Jd = J < D ? J : D;
     hipLaunchKernelGGL((k_eval<D, LT, Jd, B, W, EXPL, T, Acc>), dim3(unsigned(blocks)),
                        dim3(block), 0, s, prm);
@@ -323,17 +339,18 @@ hipError_t launch_one(const Params<T>& prm, int block, hipStream_t s) {
 // Instantiated per dimension range in sg_eval_inst.hip (one object per range and
 // precision, compiled in parallel).
 template <int D, bool EXPL, class T, class Acc>
-hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int block, hipStream_t s)
+hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int64_t blocks, int block,
+                          hipStream_t s)
 #ifdef SG_EVAL_INSTANTIATE
 {
-    if (lt > 8) return launch_one<D, 16, 0, 8, 0, EXPL, T, Acc>(prm, block, s);
+    if (lt > 8) return launch_one<D, 16, 0, 8, 0, EXPL, T, Acc>(prm, blocks, block, s);
     if constexpr (EXPL) {
-        return launch_one<D, 8, kExplicitJ, 8, 3, true, T, Acc>(prm, block, s);
+        return launch_one<D, 8, kExplicitJ, 8, 3, true, T, Acc>(prm, blocks, block, s);
     } else {
         switch (variant) {
-            case 1: return launch_one<D, 8, 2, 8, 3, false, T, Acc>(prm, block, s);
-            case 2: return launch_one<D, 8, 3, 4, 4, false, T, Acc>(prm, block, s);
-            default: return launch_one<D, 8, 3, 8, 3, false, T, Acc>(prm, block, s);
+            case 1: return launch_one<D, 8, 2, 8, 3, false, T, Acc>(prm, blocks, block, s);
+            case 2: return launch_one<D, 8, 3, 4, 4, false, T, Acc>(prm, blocks, block, s);
+            default: return launch_one<D, 8, 3, 8, 3, false, T, Acc>(prm, blocks, block, s);
         }
     }
 }
@@ -342,10 +359,11 @@ hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int block, 
 #endif
 
 template <bool EXPL, class T, class Acc>
-hipError_t launch_d(int d, int variant, int lt, const Params<T>& prm, int block, hipStream_t s) {
+hipError_t launch_d(int d, int variant, int lt, const Params<T>& prm, int64_t blocks, int block,
+                    hipStream_t s) {
     switch (d) {
 #define SG_D(DD) \
-    case DD: return launch_variant<DD, EXPL, T, Acc>(variant, lt, prm, block, s);
+    case DD: return launch_variant<DD, EXPL, T, Acc>(variant, lt, prm, blocks, block, s);
         SG_D(1) SG_D(2) SG_D(3) SG_D(4) SG_D(5) SG_D(6) SG_D(7) SG_D(8) SG_D(9) SG_D(10)
         SG_D(11) SG_D(12) SG_D(13) SG_D(14) SG_D(15) SG_D(16) SG_D(17) SG_D(18) SG_D(19)
         SG_D(20)
@@ -357,12 +375,17 @@ hipError_t launch_d(int d, int variant, int lt, const Params<T>& prm, int block,
 }  // namespace eval
 }  // namespace sg
 
-#include <string>
-
-#include "sg_kernels.hpp"
-
 namespace sg {
 namespace eval {
+
+// Walk variant and table length for grid g: the J tabled dims need levels below 8.
+inline int table_levels(const Grid& g, int variant) {
+    if (g.explicit_set) return explicit_tab_dims(g) == 0 ? 16 : 8;
+    const int J = kVariants[variant].J < g.d ? kVariants[variant].J : g.d;
+    for (int j = 0; j < J; ++j)
+        if (g.limits[j] > 7) return 16;
+    return 8;
+}
 
 template <class T, class Acc>
 int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const T* coords,
@@ -381,7 +404,8 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
         set_error("evaluate: surpluses beyond 2 GiB");
         return kRange;
     }
-    if ((P + tu.block - 1) / tu.block > int64_t(0x7fffffff)) {
+    const int64_t blocks = (P + tu.block - 1) / tu.block;
+    if (blocks > int64_t(0x7fffffff)) {
         set_error("evaluate: too many points for one launch");
         return kRange;
     }
@@ -389,24 +413,14 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
     prm.alpha = alpha;
     prm.coords = coords;
     prm.out = out;
-    prm.P = P;
-    prm.leaf_base = t.dfs_base;
-    prm.bounds = t.dfs_bound;
+    prm.grids = static_cast<const GridDesc*>(t.desc);
+    prm.work = nullptr;
     prm.perm = perm;
-    prm.alpha_bytes = uint32_t(g.N * int64_t(sizeof(T)));
-    prm.n = g.n;
-    for (int j = 0; j < g.d; ++j) prm.limits[j] = g.limits[j];
-    int lt = 8;
-    hipError_t e;
-    if (g.explicit_set) {
-        if (explicit_tab_dims(g) == 0) lt = 16;   // no tables: no blocks in the codes
-        e = launch_d<true, T, Acc>(g.d, 0, lt, prm, tu.block, s);
-    } else {
-        const int J = kVariants[tu.variant].J < g.d ? kVariants[tu.variant].J : g.d;
-        for (int j = 0; j < J; ++j)
-            if (g.limits[j] > 7) lt = 16;
-        e = launch_d<false, T, Acc>(g.d, tu.variant, lt, prm, tu.block, s);
-    }
+    prm.P = P;
+    const hipError_t e = g.explicit_set
+        ? launch_d<true, T, Acc>(g.d, 0, table_levels(g, 0), prm, blocks, tu.block, s)
+        : launch_d<false, T, Acc>(g.d, tu.variant, table_levels(g, tu.variant), prm, blocks,
+                                  tu.block, s);
     if (e != hipSuccess) {
         set_error(std::string("k_eval launch: ") + hipGetErrorString(e));
         return kHip;

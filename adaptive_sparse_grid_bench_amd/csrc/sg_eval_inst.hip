@@ -15,11 +15,11 @@ using TT = float;
 using AA = AccKahan;
 #endif
 
-#define SG_INST(DD)                                                                          \
-    template hipError_t launch_variant<DD, false, TT, AA>(int, int, const Params<TT>&, int,  \
-                                                          hipStream_t);                      \
-    template hipError_t launch_variant<DD, true, TT, AA>(int, int, const Params<TT>&, int,   \
-                                                         hipStream_t);
+#define SG_INST(DD)                                                                        \
+    template hipError_t launch_variant<DD, false, TT, AA>(int, int, const Params<TT>&,     \
+                                                          int64_t, int, hipStream_t);      \
+    template hipError_t launch_variant<DD, true, TT, AA>(int, int, const Params<TT>&,      \
+                                                         int64_t, int, hipStream_t);
 #define SG_IN_RANGE(DD) (SG_DMIN <= DD && DD <= SG_DMAX)
 
 #if SG_IN_RANGE(1)

@@ -294,14 +294,8 @@ int hierarchize_impl(const Grid& g, const DeviceTables& t, T* v, hipStream_t s) 
     for (int k = 0; k < g.d; ++k) {
         const DeviceTables::Dim& dm = t.dims[k];
         if (dm.num_tiles == 0) continue;
-        size_t lds = size_t(dm.tile_values) * sizeof(T) + size_t(dm.max_families) * 4;
-        if (lds > 64 * 1024) {
-            set_error("hierarchize: pole-family tile exceeds 64 KiB of LDS");
-            return kUnsupported;
-        }
-        hipLaunchKernelGGL(k_hierarchize_dim<T>, dim3(dm.num_tiles), dim3(kThreads), lds, s, v,
-                           dm.tiles, dm.fams, dm.blocks, dm.tile_values);
-        int st = check_launch("k_hierarchize_dim");
+        int st = launch_hierarchize_tiles<T>(v, dm.tiles, dm.fams, dm.blocks, dm.num_tiles,
+                                             dm.tile_values, dm.max_families, s);
         if (st != kOk) return st;
     }
     return kOk;
@@ -509,6 +503,24 @@ int upload(T*& dst, const T* src, size_t count) {
 
 }  // namespace
 
+template <class T>
+int launch_hierarchize_tiles(T* v, const int32_t* tiles, const int32_t* fams,
+                             const int32_t* blocks, int num_tiles, int tile_values,
+                             int max_families, hipStream_t s) {
+    const size_t lds = size_t(tile_values) * sizeof(T) + size_t(max_families) * 4;
+    if (lds > 64 * 1024) {
+        set_error("hierarchize: pole-family tile exceeds 64 KiB of LDS");
+        return kUnsupported;
+    }
+    hipLaunchKernelGGL(k_hierarchize_dim<T>, dim3(num_tiles), dim3(kThreads), lds, s, v, tiles,
+                       fams, blocks, tile_values);
+    return check_launch("k_hierarchize_dim");
+}
+template int launch_hierarchize_tiles<double>(double*, const int32_t*, const int32_t*,
+                                              const int32_t*, int, int, int, hipStream_t);
+template int launch_hierarchize_tiles<float>(float*, const int32_t*, const int32_t*,
+                                             const int32_t*, int, int, int, hipStream_t);
+
 // ------------------------------------------------------------------- launchers --
 
 int upload_tables(const Grid& g, DeviceTables& t) {
@@ -526,6 +538,15 @@ int upload_tables(const Grid& g, DeviceTables& t) {
     if ((st = upload(t.sub_base, g.sub_base.data(), g.sub_base.size())) != kOk) return st;
     if ((st = upload(t.dfs_base, g.dfs_base.data(), g.dfs_base.size())) != kOk) return st;
     if ((st = upload(t.dfs_bound, g.dfs_bound.data(), g.dfs_bound.size())) != kOk) return st;
+    eval::GridDesc desc{};
+    desc.leaf_base = t.dfs_base;
+    desc.bounds = t.dfs_bound;
+    desc.N = g.N;
+    desc.n = g.n;
+    for (int j = 0; j < g.d; ++j) desc.limits[j] = g.limits[j];
+    eval::GridDesc* ddesc = nullptr;
+    if ((st = upload(ddesc, &desc, 1)) != kOk) return st;
+    t.desc = ddesc;
     for (int k = 0; k < g.d; ++k) {
         const HierTables& h = g.hier[k];
         DeviceTables::Dim& dm = t.dims[k];
@@ -552,6 +573,8 @@ void free_tables(DeviceTables& t) {
     free_dev(t.sub_base);
     free_dev(t.dfs_base);
     free_dev(t.dfs_bound);
+    if (t.desc) (void)hipFree(t.desc);
+    t.desc = nullptr;
     for (auto& dm : t.dims) {
         free_dev(dm.tiles);
         free_dev(dm.fams);

@@ -11,6 +11,30 @@
 
 namespace sg {
 
+namespace eval {
+// One grid of a launch: where its surpluses, points and outputs start in the launch's
+// buffers, its walk tables and its level-set bounds.  Device-resident.
+struct GridDesc {
+    int64_t alpha_off;          // element offset of the grid's surpluses
+    int64_t coord_off;          // element offset of its first point's coordinates
+    int64_t out_off;            // element offset of its first output
+    const int32_t* leaf_base;   // subspace base offsets in walk order
+    const uint8_t* bounds;      // explicit sets: one code per walk node
+    int64_t N;                  // grid points (the buffer range of its surpluses)
+    int32_t n;                  // level sums 0..n-1
+    int32_t limits[kMaxDims];
+};
+
+// A run of at most one workgroup's points of one grid.
+struct WorkItem {
+    int32_t grid;
+    int32_t first;              // first point within the grid
+    int32_t count;              // points (<= threads per workgroup)
+    int32_t pad;
+};
+
+}  // namespace eval
+
 // Device copy of the tables one grid's kernels read.  Owned by the grid handle.
 struct DeviceTables {
     int device = -1;
@@ -21,6 +45,7 @@ struct DeviceTables {
     int32_t* sub_base = nullptr;    // [S]
     int32_t* dfs_base = nullptr;    // [S]
     uint8_t* dfs_bound = nullptr;   // explicit sets: walk-node bounds
+    void* desc = nullptr;           // the grid's evaluation descriptor (eval::GridDesc)
     struct Dim {
         int32_t* tiles = nullptr;
         int32_t* fams = nullptr;
@@ -42,6 +67,11 @@ int launch_gp2idx(const Grid& g, const DeviceTables& t, const int32_t* levels,
 int launch_grid_coords_f64(const Grid& g, const DeviceTables& t, double* coords, hipStream_t s);
 
 int launch_hierarchize_f64(const Grid& g, const DeviceTables& t, double* v, hipStream_t s);
+// one dimension sweep over a set of pole-family tiles (of one grid or of a batch)
+template <class T>
+int launch_hierarchize_tiles(T* v, const int32_t* tiles, const int32_t* fams,
+                             const int32_t* blocks, int num_tiles, int tile_values,
+                             int max_families, hipStream_t s);
 int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipStream_t s);
 
 struct EvalTuning {

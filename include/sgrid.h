@@ -152,6 +152,37 @@ int sg_evaluate_ref_f64(const sg_grid *grid, const double *d_alpha, const double
 int sg_evaluate_ref_f32(const sg_grid *grid, const float *d_alpha, const float *d_coords,
                         long long num_evals, float *d_out, void *stream);
 
+/* ---------------------------------------------------------- batches --------- */
+/* A batch of independent grids resident together (the batched form).  Surpluses of
+ * grid g live at value_offsets[g] of one buffer; after sg_batch_set_points(counts),
+ * grid g's points are row-major [counts[g]][d_g] starting at element coord_offsets[g]
+ * of one coordinate buffer and its outputs at point_offsets[g] of one output buffer.
+ * Hierarchization runs one launch per dimension sweep over the pole-family tiles of
+ * every grid; evaluation one launch per group of grids with the same dimensions, over
+ * device-resident work lists.  sg_batch_run_graph captures that sequence once per
+ * buffer triple into a hipGraph (evaluation groups on parallel branches) and launches
+ * it; later calls with the same buffers relaunch the same graph.  The grids must stay
+ * alive while the batch exists.  Results are bit-identical to the per-grid calls. */
+typedef struct sg_batch sg_batch;
+int sg_batch_create(sg_grid *const *grids, int count, sg_batch **out);
+int sg_batch_destroy(sg_batch *batch);
+/* total grid points and value_offsets[count+1] (may be NULL) */
+int sg_batch_values(const sg_batch *batch, long long *total, long long *value_offsets);
+/* per-grid point counts; point_offsets[count+1] and coord_offsets[count+1] out (NULL ok) */
+int sg_batch_set_points(sg_batch *batch, const long long *counts, long long *point_offsets,
+                        long long *coord_offsets);
+int sg_batch_hierarchize_f64(sg_batch *batch, double *d_values, void *stream);
+int sg_batch_hierarchize_f32(sg_batch *batch, float *d_values, void *stream);
+int sg_batch_evaluate_f64(sg_batch *batch, const double *d_alpha, const double *d_coords,
+                          double *d_out, void *stream);
+int sg_batch_evaluate_f32c(sg_batch *batch, const float *d_alpha, const float *d_coords,
+                           float *d_out, void *stream);
+/* hierarchize (if with_hierarchize) then evaluate, as one hipGraph launch */
+int sg_batch_run_graph_f64(sg_batch *batch, double *d_alpha, const double *d_coords,
+                           double *d_out, int with_hierarchize, void *stream);
+int sg_batch_run_graph_f32c(sg_batch *batch, float *d_alpha, const float *d_coords,
+                            float *d_out, int with_hierarchize, void *stream);
+
 /* ----------------------------------------------------------- inputs --------- */
 
 /* count values k * 2^-24, k = top 24 bits of splitmix64(seed * 2^32 + j), j = 0..count-1:

@@ -36,6 +36,9 @@ EXPORTS = [
     "sg_hierarchize_f64", "sg_hierarchize_f32",
     "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
     "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
+    "sg_batch_create", "sg_batch_destroy", "sg_batch_values", "sg_batch_set_points",
+    "sg_batch_hierarchize_f64", "sg_batch_hierarchize_f32", "sg_batch_evaluate_f64",
+    "sg_batch_evaluate_f32c", "sg_batch_run_graph_f64", "sg_batch_run_graph_f32c",
 ]
 
 
@@ -101,6 +104,16 @@ def _load() -> C.CDLL:
         "sg_uniform_points_f64": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_uniform_points_f32": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_convert_f64_to_f32": (i, [vp, vp, ll, vp]),
+        "sg_batch_create": (i, [C.POINTER(vp), i, C.POINTER(vp)]),
+        "sg_batch_destroy": (i, [vp]),
+        "sg_batch_values": (i, [vp, llp, llp]),
+        "sg_batch_set_points": (i, [vp, llp, llp, llp]),
+        "sg_batch_hierarchize_f64": (i, [vp, vp, vp]),
+        "sg_batch_hierarchize_f32": (i, [vp, vp, vp]),
+        "sg_batch_evaluate_f64": (i, [vp, vp, vp, vp, vp]),
+        "sg_batch_evaluate_f32c": (i, [vp, vp, vp, vp, vp]),
+        "sg_batch_run_graph_f64": (i, [vp, vp, vp, vp, i, vp]),
+        "sg_batch_run_graph_f32c": (i, [vp, vp, vp, vp, i, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -279,3 +292,68 @@ def uniform_points(out, seed: int = 42, stream=None) -> None:
 
 def to_f32(src, dst, stream=None) -> None:
     check(lib.sg_convert_f64_to_f32(_ptr(src), _ptr(dst), src.numel(), _stream(stream)))
+
+
+class Batch:
+    """Independent grids resident together (sg_batch in include/sgrid.h).
+
+    ``values`` buffers hold grid g's surpluses at ``value_offsets[g]``; after
+    ``set_points(counts)`` coordinates of grid g start at element ``coord_offsets[g]``
+    and its outputs at ``point_offsets[g]``.
+    """
+
+    def __init__(self, grids):
+        self.grids = list(grids)                 # keep the grids alive
+        arr = (C.c_void_p * len(self.grids))(*[g._h.value if isinstance(g._h, C.c_void_p)
+                                                 else g._h for g in self.grids])
+        h = C.c_void_p()
+        check(lib.sg_batch_create(arr, len(self.grids), C.byref(h)))
+        self._h = h
+        total = C.c_longlong()
+        offs = np.zeros(len(self.grids) + 1, np.int64)
+        check(lib.sg_batch_values(self._h, C.byref(total),
+                                  offs.ctypes.data_as(C.POINTER(C.c_longlong))))
+        self.total_values = total.value
+        self.value_offsets = offs
+        self.point_offsets = None
+        self.coord_offsets = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.sg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_points(self, counts):
+        c = np.ascontiguousarray(counts, np.int64)
+        po = np.zeros(len(self.grids) + 1, np.int64)
+        co = np.zeros(len(self.grids) + 1, np.int64)
+        llp = C.POINTER(C.c_longlong)
+        check(lib.sg_batch_set_points(self._h, c.ctypes.data_as(llp), po.ctypes.data_as(llp),
+                                      co.ctypes.data_as(llp)))
+        self.point_offsets, self.coord_offsets = po, co
+
+    def hierarchize(self, values, stream=None):
+        import torch
+        fn = lib.sg_batch_hierarchize_f64 if values.dtype == torch.float64 else \
+            lib.sg_batch_hierarchize_f32
+        check(fn(self._h, _ptr(values), _stream(stream)))
+
+    def evaluate(self, alpha, coords, out, stream=None):
+        import torch
+        fn = lib.sg_batch_evaluate_f64 if alpha.dtype == torch.float64 else \
+            lib.sg_batch_evaluate_f32c
+        check(fn(self._h, _ptr(alpha), _ptr(coords), _ptr(out), _stream(stream)))
+
+    def run_graph(self, alpha, coords, out, hierarchize=True, stream=None):
+        """Hierarchize (in place) then evaluate as one hipGraph launch."""
+        import torch
+        fn = lib.sg_batch_run_graph_f64 if alpha.dtype == torch.float64 else \
+            lib.sg_batch_run_graph_f32c
+        check(fn(self._h, _ptr(alpha), _ptr(coords), _ptr(out), int(hierarchize),
+                 _stream(stream)))

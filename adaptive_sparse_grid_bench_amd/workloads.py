@@ -150,3 +150,16 @@ def adaptive_level_set(d: int = 8, base_n: int = 3, extra: int = 4, seed: int = 
         current.add(m)
         push_neighbours(m)      # every newly admissible set member has m as a parent
     return np.array(sorted(current), np.uint8), c
+
+
+# --- the batched workload ---------------------------------------------------------------
+
+def batched_specs(count: int = 256, seed: int = 5):
+    """Per grid: d ~ U{2..12}, n ~ U{3..8}, c_g ~ U[0, 1), points floor(2^U[12, 18])."""
+    u = uniform_u24(count * 4, seed).astype(np.float64) * 2.0 ** -24
+    u = u.reshape(count, 4)
+    d = 2 + np.floor(u[:, 0] * 11).astype(int)
+    n = 3 + np.floor(u[:, 1] * 6).astype(int)
+    c = u[:, 2]
+    P = np.floor(2.0 ** (12 + 6 * u[:, 3])).astype(np.int64)
+    return [dict(d=int(d[i]), n=int(n[i]), c=float(c[i]), P=int(P[i])) for i in range(count)]

@@ -109,7 +109,7 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
     }
     // descriptors and work lists, grouped by (explicit set, dimensions, table length)
     std::vector<eval::GridDesc> desc(G);
-    std::map<std::tuple<int, int, int>, std::vector<eval::WorkItem>> groups_h;
+    std::map<std::tuple<int, int, int, int, int>, std::vector<eval::WorkItem>> groups_h;
     for (int i = 0; i < G; ++i) {
         const Grid& g = *grids[i].g;
         eval::GridDesc& D = desc[i];
@@ -123,7 +123,8 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
         for (int j = 0; j < g.d; ++j) D.limits[j] = g.limits[j];
         // table length valid for every variant (variant 0 tables the most dimensions)
         const int lt = eval::table_levels(g, 0);
-        auto& items = groups_h[std::make_tuple(g.explicit_set ? 1 : 0, g.d, lt)];
+        auto& items = groups_h[std::make_tuple(g.explicit_set ? 1 : 0, g.d, lt,
+                                               grids[i].variant_f64, grids[i].variant_f32)];
         for (int64_t f = 0; f < counts[i]; f += eval::kThreads)
             items.push_back({i, int32_t(f), int32_t(std::min<int64_t>(eval::kThreads, counts[i] - f)), 0});
     }
@@ -134,6 +135,8 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
         gr.explicit_set = std::get<0>(kv.first) != 0;
         gr.d = std::get<1>(kv.first);
         gr.lt = std::get<2>(kv.first);
+        gr.variant_f64 = std::get<3>(kv.first);
+        gr.variant_f32 = std::get<4>(kv.first);
         gr.first = int64_t(all.size());
         gr.count = int64_t(kv.second.size());
         all.insert(all.end(), kv.second.begin(), kv.second.end());
@@ -168,7 +171,7 @@ int Batch::evaluate_group(const Group& gr, const T* alpha, const T* coords, T* o
     prm.work = d_work + gr.first;
     prm.perm = nullptr;
     prm.P = 0;
-    const int variant = gr.explicit_set ? 0 : default_variant(gr.d, sizeof(T) == 4);
+    const int variant = gr.explicit_set ? 0 : (sizeof(T) == 4 ? gr.variant_f32 : gr.variant_f64);
     hipError_t e = gr.explicit_set
         ? eval::launch_d<true, T, Acc>(gr.d, 0, gr.lt, prm, gr.count, eval::kThreads, s)
         : eval::launch_d<false, T, Acc>(gr.d, variant, gr.lt, prm, gr.count, eval::kThreads, s);

@@ -18,6 +18,8 @@ struct Batch {
     struct GridRef {
         const Grid* g;
         const DeviceTables* t;
+        int variant_f64;        // the grid's tuned walk variants
+        int variant_f32;
     };
     struct HierDim {
         int32_t* tiles = nullptr;
@@ -31,6 +33,8 @@ struct Batch {
         bool explicit_set;
         int d;
         int lt;
+        int variant_f64;
+        int variant_f32;
         int64_t first;           // first work item
         int64_t count;           // work items (= workgroups)
     };

@@ -10,6 +10,7 @@
 #include "../../include/sgrid.h"
 #include "sg_batch.hpp"
 #include "sg_grid.hpp"
+#include "sg_server.hpp"
 #include "sg_kernels.hpp"
 
 // Device tables are uploaded on the first device call, to the device current then, so
@@ -364,7 +365,8 @@ int sg_batch_create(sg_grid* const* grids, int count, sg_batch** out) {
         if (!grids[i]) return fail(sg::kInvalid, "sg_batch_create: NULL grid");
         int st = check_device(grids[i]);           // uploads the grid's tables
         if (st != sg::kOk) return st;
-        refs.push_back({&grids[i]->g, &grids[i]->t});
+        refs.push_back({&grids[i]->g, &grids[i]->t, grids[i]->tune.variant,
+                        grids[i]->tune32.variant});
     }
     sg_batch* b = new (std::nothrow) sg_batch();
     if (!b) return fail(sg::kNoMem, "sg_batch_create: host allocation failed");
@@ -461,6 +463,89 @@ int sg_batch_run_graph_f32c(sg_batch* batch, float* d_alpha, const float* d_coor
     if (st != sg::kOk) return st;
     return batch->b.run_graph_f32c(d_alpha, d_coords, d_out, with_hierarchize != 0,
                                    as_stream(stream));
+}
+
+}  // extern "C"
+
+struct sg_server {
+    sg::Server s;
+    int precision = SG_SERVE_F64;
+};
+
+extern "C" {
+
+int sg_server_create(const sg_server_config* cfg, sg_server** out) {
+    if (!cfg || !out) return fail(sg::kInvalid, "sg_server_create: NULL argument");
+    *out = nullptr;
+    if (cfg->precision != SG_SERVE_F64 && cfg->precision != SG_SERVE_F32C)
+        return fail(sg::kInvalid, "sg_server_create: unknown precision");
+    sg::ServerConfig c;
+    c.elem_bytes = cfg->precision == SG_SERVE_F64 ? 8 : 4;
+    c.slots = cfg->slots;
+    c.max_batch_points = cfg->max_batch_points;
+    c.max_requests = cfg->max_requests;
+    c.budget_bytes = cfg->budget_bytes;
+    if (c.budget_bytes <= 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+            return fail(sg::kHip, "sg_server_create: hipMemGetInfo failed");
+        c.budget_bytes = int64_t(free_b / 4);
+    }
+    sg_server* h = new (std::nothrow) sg_server();
+    if (!h) return fail(sg::kNoMem, "sg_server_create: host allocation failed");
+    h->precision = cfg->precision;
+    int st = h->s.start(c);
+    if (st != sg::kOk) {
+        delete h;
+        return st;
+    }
+    *out = h;
+    return sg::kOk;
+}
+
+int sg_server_destroy(sg_server* server) {
+    delete server;
+    return sg::kOk;
+}
+
+int sg_server_add_grid(sg_server* server, const sg_grid* grid, const void* d_alpha,
+                       int* grid_id) {
+    if (!server || !grid || !d_alpha || !grid_id)
+        return fail(sg::kInvalid, "sg_server_add_grid: NULL argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return server->s.add_grid(&grid->g, &grid->t, d_alpha, grid_id);
+}
+
+int sg_server_submit(sg_server* server, int grid_id, const void* points, long long num_points,
+                     void* out, long long* request_id) {
+    if (!server || !request_id) return fail(sg::kInvalid, "sg_server_submit: NULL argument");
+    int64_t id = 0;
+    int st = server->s.submit(grid_id, points, num_points, out, &id);
+    *request_id = id;
+    return st;
+}
+
+int sg_server_poll(sg_server* server, long long request_id, int* done) {
+    if (!server || !done) return fail(sg::kInvalid, "sg_server_poll: NULL argument");
+    return server->s.poll(request_id, done);
+}
+
+int sg_server_wait(sg_server* server, long long request_id, double* latency_seconds) {
+    if (!server) return fail(sg::kInvalid, "sg_server_wait: NULL server");
+    return server->s.wait(request_id, latency_seconds);
+}
+
+int sg_server_stats(sg_server* server, long long* launches, long long* points_done,
+                    long long* waiting, long long* budget_used) {
+    if (!server) return fail(sg::kInvalid, "sg_server_stats: NULL server");
+    int64_t a = 0, b = 0, c = 0, d = 0;
+    int st = server->s.stats(&a, &b, &c, &d);
+    if (launches) *launches = a;
+    if (points_done) *points_done = b;
+    if (waiting) *waiting = c;
+    if (budget_used) *budget_used = d;
+    return st;
 }
 
 }  // extern "C"

@@ -308,8 +308,9 @@ void k_eval(Params<T> prm) {
 #pragma unroll
     for (int k = 0; k < D; ++k) lim[k] = G.limits[k];
     // buffer descriptor word 3 for gfx9-family raw (untyped) buffer access
-    State<T, Acc> st{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(prm.alpha + G.alpha_off),
-                                                       short(0), int(G.N * int64_t(sizeof(T))),
+    const T* alpha = prm.alpha ? prm.alpha + G.alpha_off : static_cast<const T*>(G.alpha_ptr);
+    State<T, Acc> st{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(alpha), short(0),
+                                                       int(G.N * int64_t(sizeof(T))),
                                                        0x00020000),
                      G.leaf_base, G.bounds, 0, 0, Acc()};
     Walker<D - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, lim, T(1), 0u, 0, G.n - 1, st);

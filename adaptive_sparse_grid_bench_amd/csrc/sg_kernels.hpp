@@ -15,7 +15,9 @@ namespace eval {
 // One grid of a launch: where its surpluses, points and outputs start in the launch's
 // buffers, its walk tables and its level-set bounds.  Device-resident.
 struct GridDesc {
-    int64_t alpha_off;          // element offset of the grid's surpluses
+    int64_t alpha_off;          // element offset of the grid's surpluses in the launch's
+                                // surplus buffer, or
+    const void* alpha_ptr;      // their address when the launch passes no buffer
     int64_t coord_off;          // element offset of its first point's coordinates
     int64_t out_off;            // element offset of its first output
     const int32_t* leaf_base;   // subspace base offsets in walk order

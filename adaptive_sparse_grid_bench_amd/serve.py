@@ -1,0 +1,180 @@
+"""The served workload: resident grids, Poisson-arrival evaluation requests, latency
+percentiles and throughput of the continuous-batching server against plain per-request
+FP64 launches.
+
+Workload: ``count`` resident grids, a mix of regular ones (d ~ U{2..16}, n ~ U{3..8},
+f = exp(-sum (x-1/2)^2)) and refined-subspace ones (d = 8, base level 3, up to 4 extra
+levels per dimension, refined towards a seeded Gaussian peak of width 0.15, f that peak).
+Requests pick a grid uniformly and carry floor(2^U[0,16]) uniform points (1 to 65,536,
+log-uniform).  Arrivals are Poisson at a rate set as a fraction of the capacity measured
+by a closed-loop run.  Latency = completion - arrival (queueing included); throughput =
+points / (last completion - first arrival).
+
+Usage: python -m adaptive_sparse_grid_bench_amd.serve [--grids 64] [--requests 600]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import threading
+import time
+
+import numpy as np
+
+from . import sgrid
+from . import workloads as W
+
+
+def make_grids(count: int = 64, seed: int = 21, adaptive_every: int = 4,
+               max_points: int = 1 << 25):
+    """Resident grids with FP64 surpluses and FP32 copies (torch CUDA tensors)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    grids = []
+    k = 0
+    while len(grids) < count:
+        k += 1
+        if len(grids) % adaptive_every == adaptive_every - 1:
+            lv, c = W.adaptive_level_set(d=8, base_n=3, extra=4, seed=seed * 1000 + k)
+            g = sgrid.Grid.explicit(8, lv)
+            kind = "adaptive"
+            centre = torch.tensor(c, dtype=torch.float64, device="cuda")
+        else:
+            d, n = int(rng.integers(2, 17)), int(rng.integers(3, 9))
+            g = sgrid.Grid(d, n)
+            if g.N > max_points:
+                continue
+            kind = "regular"
+            centre = None
+        x = torch.empty((g.N, g.d), dtype=torch.float64, device="cuda")
+        g.grid_coords(x)
+        if centre is None:
+            a = torch.exp(-((x - 0.5) ** 2).sum(1))
+        else:
+            a = torch.exp(-((x - centre) ** 2).sum(1) / (2 * 0.15 ** 2))
+        del x
+        g.hierarchize(a)
+        grids.append({"grid": g, "a64": a, "a32": a.float(), "kind": kind})
+    torch.cuda.synchronize()
+    return grids
+
+
+def make_requests(grids, count: int, seed: int = 5):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for r in range(count):
+        gi = int(rng.integers(len(grids)))
+        P = int(np.floor(2.0 ** rng.uniform(0, 16)))
+        pts = W.uniform_points(P, grids[gi]["grid"].d, seed=100000 + r)
+        reqs.append((gi, pts, pts.astype(np.float32)))
+    return reqs
+
+
+def _percentiles(lat):
+    a = np.sort(np.asarray(lat))
+    return float(np.percentile(a, 50)), float(np.percentile(a, 99))
+
+
+def run_server(grids, reqs, precision: str, arrivals=None, slots: int = 3,
+               max_batch_points: int = 1 << 20):
+    """Serve ``reqs`` (all at once if ``arrivals`` is None, else at those offsets in s)."""
+    srv = sgrid.Server(precision, slots=slots, max_batch_points=max_batch_points,
+                       max_requests=2048)
+    key = "a64" if precision == "f64" else "a32"
+    ids = [srv.add_grid(g["grid"], g[key]) for g in grids]
+    dt = np.float64 if precision == "f64" else np.float32
+    outs = [np.empty(len(p64), dt) for _, p64, _ in reqs]
+    rids = [None] * len(reqs)
+    t_sub = [0.0] * len(reqs)
+    ready = threading.Event()
+    t0 = [0.0]
+
+    def client():
+        ready.wait()
+        for i, (gi, p64, p32) in enumerate(reqs):
+            if arrivals is not None:
+                delay = t0[0] + arrivals[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            t_sub[i] = time.perf_counter()
+            rids[i] = srv.submit(ids[gi], p64 if precision == "f64" else p32, outs[i])
+
+    th = threading.Thread(target=client)
+    th.start()
+    t0[0] = time.perf_counter()
+    ready.set()
+    th.join()
+    lat = [srv.wait(r) for r in rids]
+    t_end = max(t_sub[i] + lat[i] for i in range(len(reqs)))
+    stats = srv.stats()
+    srv.close()
+    pts = sum(len(o) for o in outs)
+    p50, p99 = _percentiles(lat)
+    return {"points": pts, "seconds": t_end - t0[0], "points_per_s": pts / (t_end - t0[0]),
+            "p50_s": p50, "p99_s": p99, "launches": stats["launches"],
+            "requests": len(reqs)}, outs
+
+
+def run_plain_f64(grids, reqs, arrivals=None):
+    """The plain FP64 path: one request at a time, copy in, per-grid evaluate, copy out."""
+    import torch
+    stream = torch.cuda.current_stream()
+    lat, outs = [], []
+    t0 = time.perf_counter()
+    t_end = t0
+    for i, (gi, p64, _) in enumerate(reqs):
+        arrive = t0 + (arrivals[i] if arrivals is not None else 0.0)
+        now = time.perf_counter()
+        if now < arrive:
+            time.sleep(arrive - now)
+        x = torch.from_numpy(p64).cuda(non_blocking=False)
+        o = torch.empty(len(p64), dtype=torch.float64, device="cuda")
+        grids[gi]["grid"].evaluate(grids[gi]["a64"], x, o, stream)
+        out = o.cpu().numpy()
+        t_end = time.perf_counter()
+        lat.append(t_end - arrive)
+        outs.append(out)
+    pts = sum(len(o) for o in outs)
+    p50, p99 = _percentiles(lat)
+    return {"points": pts, "seconds": t_end - t0, "points_per_s": pts / (t_end - t0),
+            "p50_s": p50, "p99_s": p99, "requests": len(reqs)}, outs
+
+
+def experiment(grids, n_requests: int = 600, load: float = 0.8, seed: int = 5):
+    """Closed-loop capacity, then Poisson arrivals at `load` x capacity for the FP32
+    Kahan server, the FP64 server and the plain FP64 launches."""
+    reqs = make_requests(grids, n_requests, seed)
+    run_server(grids, reqs[: min(50, len(reqs))], "f32c")          # warm-up
+    cap, _ = run_server(grids, reqs, "f32c")
+    rate = load * cap["requests"] / cap["seconds"]                   # requests / s
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    f32, o32 = run_server(grids, reqs, "f32c", arrivals)
+    f64, o64 = run_server(grids, reqs, "f64", arrivals)
+    plain, op = run_plain_f64(grids, reqs, arrivals)
+    err = max(float(np.max(np.abs(a.astype(np.float64) - b))) if len(a) else 0.0
+              for a, b in zip(o32, o64))
+    same = all(np.array_equal(a, b) for a, b in zip(o64, op))
+    return {"capacity_closed_loop": cap, "arrival_rate_per_s": rate, "load": load,
+            "f32c_server": f32, "f64_server": f64, "plain_f64": plain,
+            "f32c_max_abs_err_vs_f64": err, "f64_server_equals_plain": same,
+            "grids": {"count": len(grids),
+                      "adaptive": sum(g["kind"] == "adaptive" for g in grids),
+                      "grid_points": int(sum(g["grid"].N for g in grids))}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--grids", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=600)
+    ap.add_argument("--load", type=float, default=0.8)
+    args = ap.parse_args()
+    t = time.time()
+    grids = make_grids(args.grids)
+    res = experiment(grids, args.requests, args.load)
+    res["setup_s"] = time.time() - t
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -39,6 +39,8 @@ EXPORTS = [
     "sg_batch_create", "sg_batch_destroy", "sg_batch_values", "sg_batch_set_points",
     "sg_batch_hierarchize_f64", "sg_batch_hierarchize_f32", "sg_batch_evaluate_f64",
     "sg_batch_evaluate_f32c", "sg_batch_run_graph_f64", "sg_batch_run_graph_f32c",
+    "sg_server_create", "sg_server_destroy", "sg_server_add_grid", "sg_server_submit",
+    "sg_server_poll", "sg_server_wait", "sg_server_stats",
 ]
 
 
@@ -53,6 +55,12 @@ class Tuning(C.Structure):
 
 
 EVAL_VARIANTS = 3
+
+
+class ServerConfig(C.Structure):
+    """sg_server_config (include/sgrid.h)."""
+    _fields_ = [("precision", C.c_int), ("slots", C.c_int), ("max_batch_points", C.c_longlong),
+                ("max_requests", C.c_int), ("budget_bytes", C.c_longlong)]
 
 
 class SgError(RuntimeError):
@@ -114,6 +122,13 @@ def _load() -> C.CDLL:
         "sg_batch_evaluate_f32c": (i, [vp, vp, vp, vp, vp]),
         "sg_batch_run_graph_f64": (i, [vp, vp, vp, vp, i, vp]),
         "sg_batch_run_graph_f32c": (i, [vp, vp, vp, vp, i, vp]),
+        "sg_server_create": (i, [C.POINTER(ServerConfig), C.POINTER(vp)]),
+        "sg_server_destroy": (i, [vp]),
+        "sg_server_add_grid": (i, [vp, vp, vp, ip]),
+        "sg_server_submit": (i, [vp, i, vp, ll, vp, llp]),
+        "sg_server_poll": (i, [vp, ll, ip]),
+        "sg_server_wait": (i, [vp, ll, C.POINTER(C.c_double)]),
+        "sg_server_stats": (i, [vp, llp, llp, llp, llp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -357,3 +372,68 @@ class Batch:
             lib.sg_batch_run_graph_f32c
         check(fn(self._h, _ptr(alpha), _ptr(coords), _ptr(out), int(hierarchize),
                  _stream(stream)))
+
+
+class Server:
+    """Evaluation server with continuous batching (sg_server in include/sgrid.h).
+
+    ``precision`` "f64" or "f32c" fixes the dtype of surpluses, points and outputs.
+    Requests carry host numpy arrays (row-major [P][d] points, [P] outputs) that are
+    kept alive here until ``wait`` returns.
+    """
+
+    def __init__(self, precision: str = "f64", slots: int = 3, max_batch_points: int = 1 << 20,
+                 max_requests: int = 1024, budget_bytes: int = 0):
+        self.dtype = np.float64 if precision == "f64" else np.float32
+        cfg = ServerConfig(0 if precision == "f64" else 1, slots, max_batch_points, max_requests,
+                           budget_bytes)
+        h = C.c_void_p()
+        check(lib.sg_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._keep = {}
+        self._grids = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.sg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_grid(self, grid: "Grid", alpha) -> int:
+        gid = C.c_int()
+        check(lib.sg_server_add_grid(self._h, grid._h, _ptr(alpha), C.byref(gid)))
+        self._grids.append((grid, alpha))
+        return gid.value
+
+    def submit(self, grid_id: int, points: np.ndarray, out: np.ndarray) -> int:
+        assert points.dtype == self.dtype and out.dtype == self.dtype
+        assert points.flags["C_CONTIGUOUS"] and out.flags["C_CONTIGUOUS"]
+        rid = C.c_longlong()
+        check(lib.sg_server_submit(self._h, grid_id, points.ctypes.data, out.shape[0],
+                                   out.ctypes.data, C.byref(rid)))
+        self._keep[rid.value] = (points, out)
+        return rid.value
+
+    def poll(self, rid: int) -> bool:
+        done = C.c_int()
+        check(lib.sg_server_poll(self._h, rid, C.byref(done)))
+        return bool(done.value)
+
+    def wait(self, rid: int) -> float:
+        """Block until request ``rid`` is done; returns its latency in seconds."""
+        lat = C.c_double()
+        try:
+            check(lib.sg_server_wait(self._h, rid, C.byref(lat)))
+        finally:
+            self._keep.pop(rid, None)
+        return lat.value
+
+    def stats(self) -> dict:
+        v = [C.c_longlong() for _ in range(4)]
+        check(lib.sg_server_stats(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("launches", "points_done", "waiting", "budget_used"), [x.value for x in v]))

@@ -162,7 +162,8 @@ int sg_evaluate_ref_f32(const sg_grid *grid, const float *d_alpha, const float *
  * device-resident work lists.  sg_batch_run_graph captures that sequence once per
  * buffer triple into a hipGraph (evaluation groups on parallel branches) and launches
  * it; later calls with the same buffers relaunch the same graph.  The grids must stay
- * alive while the batch exists.  Results are bit-identical to the per-grid calls. */
+ * alive while the batch exists; each grid's walk variants are taken from its tuning
+ * when the batch is created.  Results are bit-identical to the per-grid calls. */
 typedef struct sg_batch sg_batch;
 int sg_batch_create(sg_grid *const *grids, int count, sg_batch **out);
 int sg_batch_destroy(sg_batch *batch);
@@ -182,6 +183,41 @@ int sg_batch_run_graph_f64(sg_batch *batch, double *d_alpha, const double *d_coo
                            double *d_out, int with_hierarchize, void *stream);
 int sg_batch_run_graph_f32c(sg_batch *batch, float *d_alpha, const float *d_coords,
                             float *d_out, int with_hierarchize, void *stream);
+
+/* ---------------------------------------------------------- serving --------- */
+/* Evaluation server with continuous batching (the served form).  Grids and their
+ * surpluses (device, in the server's precision) are registered once and stay
+ * resident.  Requests (grid, host points [P][d], host output [P]) may be submitted
+ * from any thread; a worker thread admits waiting requests in arrival order while
+ * their staging bytes (points + outputs) fit the device budget, packs them into one
+ * launch per free slot (a work list across grids and requests, one evaluation kernel
+ * per dimension count), stages points through pinned memory and copies results back
+ * asynchronously.  Requests arriving while launches run join the next launch as soon
+ * as a slot and budget free up; up to `slots` launches overlap on the device.  Each
+ * request's results are the same alone and in any mix (every point runs the same
+ * walk).  Host buffers must stay valid until sg_server_wait returns; every request is
+ * waited for exactly once. */
+#define SG_SERVE_F64 0
+#define SG_SERVE_F32C 1
+typedef struct sg_server sg_server;
+typedef struct sg_server_config {
+    int precision;              /* SG_SERVE_F64 or SG_SERVE_F32C (points, outputs, surpluses) */
+    int slots;                  /* launches in flight at once, 1..16 */
+    long long max_batch_points; /* points per launch (>= 256); also the largest request */
+    int max_requests;           /* requests per launch */
+    long long budget_bytes;     /* device bytes admitted requests may hold; <= 0: a quarter
+                                   of the free device memory at creation */
+} sg_server_config;
+int sg_server_create(const sg_server_config *cfg, sg_server **out);
+int sg_server_destroy(sg_server *server);   /* finishes the requests in flight */
+int sg_server_add_grid(sg_server *server, const sg_grid *grid, const void *d_alpha,
+                       int *grid_id);
+int sg_server_submit(sg_server *server, int grid_id, const void *points, long long num_points,
+                     void *out, long long *request_id);
+int sg_server_poll(sg_server *server, long long request_id, int *done);
+int sg_server_wait(sg_server *server, long long request_id, double *latency_seconds);
+int sg_server_stats(sg_server *server, long long *launches, long long *points_done,
+                    long long *waiting, long long *budget_used);
 
 /* ----------------------------------------------------------- inputs --------- */
 

@@ -95,11 +95,20 @@ __device__ __forceinline__ T gather_leaf(__amdgpu_buffer_rsrc_t r, int32_t base,
     return gather(r, voff, uint32_t(base) * uint32_t(sizeof(T)), static_cast<T*>(nullptr));
 }
 
+// Read-only tables (work items, grid descriptors, walk-order leaf bases, node codes)
+// are read through the constant address space: with a wave-uniform address the loads
+// become scalar loads, so a gather's subspace base stays in a scalar register.
+template <class V>
+using cptr = const __attribute__((address_space(4))) V*;
+
+template <class V>
+__device__ __forceinline__ cptr<V> as_const(const V* p) { return (cptr<V>)p; }
+
 template <class T, class Acc>
 struct State {
     __amdgpu_buffer_rsrc_t alpha;
-    const int32_t* leaf_base;
-    const uint8_t* bounds;
+    cptr<int32_t> leaf_base;
+    cptr<uint8_t> bounds;
     int q;                      // next leaf (uniform)
     int bq;                     // next node code (uniform; explicit sets)
     Acc acc;
@@ -170,7 +179,7 @@ __device__ __forceinline__ T leaf_weight(const Ln& ln, T w) {
 
 template <int J, int R, int B, int c, int cend, class Ln, class T>
 __device__ __forceinline__ void block_issue(T (&a)[B], const Ln& ln, uint32_t off, int hsum,
-                                            const int32_t* bases, __amdgpu_buffer_rsrc_t alpha) {
+                                            cptr<int32_t> bases, __amdgpu_buffer_rsrc_t alpha) {
     if constexpr (c < cend) {
         a[c % B] = gather_leaf<T>(alpha, bases[c], leaf_offset<J, R, c, J - 1>(ln, off, hsum));
         block_issue<J, R, B, c + 1, cend>(a, ln, off, hsum, bases, alpha);
@@ -187,7 +196,7 @@ __device__ __forceinline__ void block_consume(const T (&a)[B], const Ln& ln, T p
 
 template <int J, int R, int B, int c0, class Ln, class T, class Acc>
 __device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, int hsum,
-                                              const int32_t* bases, State<T, Acc>& st) {
+                                              cptr<int32_t> bases, State<T, Acc>& st) {
     constexpr int C = Simplex<J, R>::C;
     if constexpr (c0 < C) {
         constexpr int cend = (c0 + B < C) ? c0 + B : C;
@@ -278,16 +287,16 @@ void k_eval(Params<T> prm) {
     int gi;
     int64_t first, count;
     if (prm.work) {
-        const WorkItem w = prm.work[blockIdx.x];
-        gi = w.grid;
-        first = w.first;
-        count = w.count;
+        const cptr<WorkItem> w = as_const(prm.work) + blockIdx.x;
+        gi = w->grid;
+        first = w->first;
+        count = w->count;
     } else {
         gi = 0;
         first = int64_t(blockIdx.x) * blockDim.x;
         count = prm.P - first < int64_t(blockDim.x) ? prm.P - first : int64_t(blockDim.x);
     }
-    const GridDesc& G = prm.grids[gi];
+    const cptr<GridDesc> G = as_const(prm.grids) + gi;
     const int64_t p = first + threadIdx.x;
     const bool live = int64_t(threadIdx.x) < count;
     // idle lanes walk the workgroup's first point and store nothing; with a
@@ -295,7 +304,7 @@ void k_eval(Params<T> prm) {
     // share cache lines
     const int64_t pc = live ? (prm.perm ? int64_t(prm.perm[p]) : p) : first;
     Lane<D, LT, J, T> ln;
-    const T* x = prm.coords + G.coord_off + pc * D;
+    const T* x = prm.coords + G->coord_off + pc * D;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         const T xk = x[k];
@@ -306,15 +315,15 @@ void k_eval(Params<T> prm) {
     }
     int lim[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) lim[k] = G.limits[k];
+    for (int k = 0; k < D; ++k) lim[k] = G->limits[k];
     // buffer descriptor word 3 for gfx9-family raw (untyped) buffer access
-    const T* alpha = prm.alpha ? prm.alpha + G.alpha_off : static_cast<const T*>(G.alpha_ptr);
+    const T* alpha = prm.alpha ? prm.alpha + G->alpha_off : static_cast<const T*>(G->alpha_ptr);
     State<T, Acc> st{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(alpha), short(0),
-                                                       int(G.N * int64_t(sizeof(T))),
+                                                       int(G->N * int64_t(sizeof(T))),
                                                        0x00020000),
-                     G.leaf_base, G.bounds, 0, 0, Acc()};
-    Walker<D - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, lim, T(1), 0u, 0, G.n - 1, st);
-    if (live) prm.out[G.out_off + pc] = st.acc.value();
+                     as_const(G->leaf_base), as_const(G->bounds), 0, 0, Acc()};
+    Walker<D - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
+    if (live) prm.out[G->out_off + pc] = st.acc.value();
 }
 
 // Compiled variants of the register-table walk (LT = 8); the autotuner picks one per

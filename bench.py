@@ -73,6 +73,38 @@ def cpu_baseline(d: int, n: int, P_sample: int, seed: int) -> dict:
                           f"at d={d} n={n} ({exc})", "host_cpu": _cpu_model()}
 
 
+def other_forms(g, alpha, pts, out64, P, args) -> dict:
+    """FP32-Kahan evaluation of the same workload, the batched form (256 grids, one
+    hipGraph launch vs per-grid launches) and the served form (continuous batching vs
+    plain FP64 per-request launches, p50/p99)."""
+    import torch
+    from adaptive_sparse_grid_bench_amd import batched, serve
+    res = {}
+    a32, p32 = alpha.float(), pts.float()
+    o32 = torch.empty(P, dtype=torch.float32, device=out64.device)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    g.evaluate(a32, p32, o32)
+    torch.cuda.synchronize()
+    ev[0].record()
+    g.evaluate(a32, p32, o32)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1])
+    err = (o32.double() - out64).abs()
+    res["f32c"] = {"evaluate_ms": ms, "pts_per_s": P / (ms * 1e-3),
+                   "max_abs_err_vs_f64": err.max().item(),
+                   "rms_err_vs_f64": err.pow(2).mean().sqrt().item()}
+    del a32, p32, o32
+    parts = batched.build(256, tune=True)
+    res["batched"] = batched.measure(*parts)
+    del parts
+    grids = serve.make_grids(64)
+    res["served"] = serve.experiment(grids, args.served_requests, 0.8)
+    del grids
+    torch.cuda.empty_cache()
+    return res
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -94,6 +126,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--cpu-sample", type=int, default=1 << 16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the FP32, batched and served forms (headline FP64 line only)")
+    ap.add_argument("--served-requests", type=int, default=600)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -211,6 +246,8 @@ def main() -> int:
         },
         "cpu_baseline": None,
     }
+    if not args.no_forms:
+        rec.update(other_forms(g, alpha, pts, out, P, args))
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(d, n, args.cpu_sample, args.seed)
     print(json.dumps(rec), flush=True)

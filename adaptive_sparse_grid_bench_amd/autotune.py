@@ -94,7 +94,8 @@ def tune(g: "sgrid.Grid", points: int = 1 << 17, reps: int = 3, precisions=("f64
         key = "eval_variant_f64" if prec == "f64" else "eval_variant_f32"
         cand = []
         for variant in range(sgrid.EVAL_VARIANTS):
-            for block in EVAL_BLOCKS:
+            # the staged walk always runs 256-thread workgroups
+            for block in ((256,) if variant == sgrid.STAGED_VARIANT else EVAL_BLOCKS):
                 for binning in (-1, 4096):
                     g.set_tuning(**{**base, key: variant, "eval_block": block,
                                     "bin_min_points": binning})

@@ -113,18 +113,16 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
     for (int i = 0; i < G; ++i) {
         const Grid& g = *grids[i].g;
         eval::GridDesc& D = desc[i];
+        D = grids[i].t->host_desc;               // walk and stage tables
         D.alpha_off = value_off[i];
+        D.alpha_ptr = nullptr;
         D.coord_off = coord_off[i];
         D.out_off = point_off[i];
-        D.leaf_base = grids[i].t->dfs_base;
-        D.bounds = grids[i].t->dfs_bound;
-        D.N = g.N;
-        D.n = g.n;
-        for (int j = 0; j < g.d; ++j) D.limits[j] = g.limits[j];
         // table length valid for every variant (variant 0 tables the most dimensions)
         const int lt = eval::table_levels(g, 0);
         auto& items = groups_h[std::make_tuple(g.explicit_set ? 1 : 0, g.d, lt,
-                                               grids[i].variant_f64, grids[i].variant_f32)];
+                                               effective_variant(g, grids[i].variant_f64, 8),
+                                               effective_variant(g, grids[i].variant_f32, 4))];
         for (int64_t f = 0; f < counts[i]; f += eval::kThreads)
             items.push_back({i, int32_t(f), int32_t(std::min<int64_t>(eval::kThreads, counts[i] - f)), 0});
     }
@@ -171,7 +169,7 @@ int Batch::evaluate_group(const Group& gr, const T* alpha, const T* coords, T* o
     prm.work = d_work + gr.first;
     prm.perm = nullptr;
     prm.P = 0;
-    const int variant = gr.explicit_set ? 0 : (sizeof(T) == 4 ? gr.variant_f32 : gr.variant_f64);
+    const int variant = sizeof(T) == 4 ? gr.variant_f32 : gr.variant_f64;
     hipError_t e = gr.explicit_set
         ? eval::launch_d<true, T, Acc>(gr.d, 0, gr.lt, prm, gr.count, eval::kThreads, s)
         : eval::launch_d<false, T, Acc>(gr.d, variant, gr.lt, prm, gr.count, eval::kThreads, s);

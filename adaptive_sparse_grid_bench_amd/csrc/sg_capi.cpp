@@ -514,7 +514,8 @@ int sg_server_add_grid(sg_server* server, const sg_grid* grid, const void* d_alp
         return fail(sg::kInvalid, "sg_server_add_grid: NULL argument");
     int st = check_device(grid);
     if (st != sg::kOk) return st;
-    return server->s.add_grid(&grid->g, &grid->t, d_alpha, grid_id);
+    const int variant = server->precision == SG_SERVE_F64 ? grid->tune.variant : grid->tune32.variant;
+    return server->s.add_grid(&grid->g, &grid->t, d_alpha, variant, grid_id);
 }
 
 int sg_server_submit(sg_server* server, int grid_id, const void* points, long long num_points,

@@ -15,7 +15,12 @@
 //   J    tabled dimensions;  B  gathers per batch;  W  waves per SIMD the register
 //        allocation targets (0 = compiler's choice);
 //   T    coordinate / surplus type;  Acc  accumulator (FP64 FMA or FP32 Kahan);
-//   EXPL explicit (refined-subspace) level set: node bounds come from a byte stream.
+//   EXPL explicit (refined-subspace) level set: node bounds come from a byte stream;
+//   STG  staged: the workgroup copies the surplus blocks of the coming leaves into LDS
+//        (buffer loads straight to LDS, 16 bytes per lane, double-buffered stages) and
+//        each lane reads its coefficient there.  Every lane of a workgroup visits the
+//        same leaves in the same order, so one copy serves all its waves; a direct
+//        gather costs one texture-address pass per wave and leaf instead.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,22 +47,30 @@ struct Params {
     int64_t P;                  // one-grid launches: number of points
 };
 
+// pin(): the sum so far must exist at this point of the program (an empty asm that
+// reads and writes it), so the compiler cannot sink a batch's accumulation below a
+// later branch and keep that batch's loaded values alive across it.
 struct AccF64 {
     double s = 0.0;
     __device__ __forceinline__ void add(double w, double a) { s = fma(w, a, s); }
     __device__ __forceinline__ double value() const { return s; }
+    __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s)); }
 };
 
-// Kahan-compensated FP32 accumulation across subspaces
+// Kahan-compensated FP32 accumulation across subspaces.  Contraction is off inside the
+// update, so every walk variant computes exactly the written compensation whatever the
+// compiler knows about s and c (at the first leaf both are constant zeros).
 struct AccKahan {
     float s = 0.0f, c = 0.0f;
     __device__ __forceinline__ void add(float w, float a) {
+#pragma clang fp contract(off)
         const float y = fmaf(w, a, -c);
         const float t = s + y;
         c = (t - s) - y;
         s = t;
     }
     __device__ __forceinline__ float value() const { return s; }
+    __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s), "+v"(c)); }
 };
 
 // Hat values and in-block byte offsets of one coordinate at levels 0..LT-1.
@@ -104,7 +117,9 @@ using cptr = const __attribute__((address_space(4))) V*;
 template <class V>
 __device__ __forceinline__ cptr<V> as_const(const V* p) { return (cptr<V>)p; }
 
-template <class T, class Acc>
+using lds_byte = __attribute__((address_space(3))) uint8_t;
+
+template <class T, class Acc, bool STG>
 struct State {
     __amdgpu_buffer_rsrc_t alpha;
     cptr<int32_t> leaf_base;
@@ -112,8 +127,54 @@ struct State {
     int q;                      // next leaf (uniform)
     int bq;                     // next node code (uniform; explicit sets)
     Acc acc;
+    // staged walk (uniform across the workgroup)
+    lds_byte* lds;              // two stage buffers of kStageBytes
+    cptr<int32_t> leaf_lds, stage_end, seg_begin, segs;
+    int stage, num_stages, qend;
+    uint32_t cur;               // byte offset of the stage being read (0 or kStageBytes)
+
+    // copy stage k's blocks into the stage buffer at byte offset buf: the waves take
+    // the segments in turn, each lane moves 16 bytes
+    __device__ __forceinline__ void issue(int k, uint32_t buf) {
+        const int s0 = seg_begin[k], s1 = seg_begin[k + 1];
+        const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+        constexpr int waves = kThreads / 64;       // staged launches use kThreads
+        const uint32_t lane = threadIdx.x & 63u;
+        for (int g = s0 + wave; g < s1; g += waves) {
+            const int src = segs[2 * g];
+            const int w1 = segs[2 * g + 1];
+            if (int(lane) < (w1 >> 16))
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    alpha, lds + buf + uint32_t(w1 & 0xffff), 16, lane * 16u, src, 0, 0);
+        }
+    }
+    // all waves are done with the current stage and the next one has landed: switch to
+    // it and start copying the one after into the buffer just released
+    __device__ __forceinline__ void advance() {
+        __syncthreads();
+        ++stage;
+        cur ^= uint32_t(kStageBytes);
+        qend = stage_end[stage];
+        if (stage + 1 < num_stages) issue(stage + 1, cur ^ uint32_t(kStageBytes));
+    }
+    // a batch of leaves starting at leaf q0 is about to be read
+    __device__ __forceinline__ void batch(int q0) {
+        if constexpr (STG) {
+            acc.pin();
+            if (q0 == qend) advance();
+        }
+    }
+    __device__ __forceinline__ T fetch(int qq, uint32_t voff) {
+        if constexpr (STG) {
+            return *reinterpret_cast<const __attribute__((address_space(3))) T*>(
+                lds + (cur + uint32_t(leaf_lds[qq]) + voff));
+        } else {
+            return gather_leaf<T>(alpha, leaf_base[qq], voff);
+        }
+    }
     __device__ __forceinline__ void leaf(T w, uint32_t voff) {
-        acc.add(w, gather_leaf<T>(alpha, leaf_base[q], voff));
+        batch(q);
+        acc.add(w, fetch(q, voff));
         ++q;
     }
 };
@@ -177,12 +238,12 @@ __device__ __forceinline__ T leaf_weight(const Ln& ln, T w) {
     return w;
 }
 
-template <int J, int R, int B, int c, int cend, class Ln, class T>
+template <int J, int R, int B, int c, int cend, class Ln, class T, class St>
 __device__ __forceinline__ void block_issue(T (&a)[B], const Ln& ln, uint32_t off, int hsum,
-                                            cptr<int32_t> bases, __amdgpu_buffer_rsrc_t alpha) {
+                                            St& st) {
     if constexpr (c < cend) {
-        a[c % B] = gather_leaf<T>(alpha, bases[c], leaf_offset<J, R, c, J - 1>(ln, off, hsum));
-        block_issue<J, R, B, c + 1, cend>(a, ln, off, hsum, bases, alpha);
+        a[c % B] = st.fetch(st.q + c, leaf_offset<J, R, c, J - 1>(ln, off, hsum));
+        block_issue<J, R, B, c + 1, cend>(a, ln, off, hsum, st);
     }
 }
 
@@ -194,31 +255,32 @@ __device__ __forceinline__ void block_consume(const T (&a)[B], const Ln& ln, T p
     }
 }
 
-template <int J, int R, int B, int c0, class Ln, class T, class Acc>
+template <int J, int R, int B, int c0, class Ln, class T, class St>
 __device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, int hsum,
-                                              cptr<int32_t> bases, State<T, Acc>& st) {
+                                              St& st) {
     constexpr int C = Simplex<J, R>::C;
     if constexpr (c0 < C) {
         constexpr int cend = (c0 + B < C) ? c0 + B : C;
         T a[B];
-        block_issue<J, R, B, c0, cend>(a, ln, off, hsum, bases, st.alpha);
+        st.batch(st.q + c0);
+        block_issue<J, R, B, c0, cend>(a, ln, off, hsum, st);
         block_consume<J, R, B, c0, cend>(a, ln, pp, st.acc);
-        block_batches<J, R, B, cend>(ln, pp, off, hsum, bases, st);
+        block_batches<J, R, B, cend>(ln, pp, off, hsum, st);
     }
 }
 
 // All C(R+J, J) leaves below a depth-J node whose remaining level budget R fits inside
 // the J tabled dimensions.
-template <int J, int R, int B, class Ln, class T, class Acc>
+template <int J, int R, int B, class Ln, class T, class St>
 __device__ __forceinline__ void static_block(const Ln& ln, T pp, uint32_t off, int hsum,
-                                             State<T, Acc>& st) {
-    block_batches<J, R, B, 0>(ln, pp, off, hsum, st.leaf_base + st.q, st);
+                                             St& st) {
+    block_batches<J, R, B, 0>(ln, pp, off, hsum, st);
     st.q += Simplex<J, R>::C;
 }
 
-template <int J, int LT, int B, class Ln, class T, class Acc>
+template <int J, int LT, int B, class Ln, class T, class St>
 __device__ __forceinline__ void static_block_dispatch(int R, const Ln& ln, T pp, uint32_t off,
-                                                      int hsum, State<T, Acc>& st) {
+                                                      int hsum, St& st) {
     switch (R) {
 #define SG_BLK(RR) \
     case RR: if constexpr (RR < LT) static_block<J, RR, B>(ln, pp, off, hsum, st); break;
@@ -238,11 +300,11 @@ __device__ __forceinline__ void static_block_dispatch(int R, const Ln& ln, T pp,
 // Node codes of an explicit set (one byte per visited node, walk order): at depth J-1
 // a code below 0x80 is a full static block of budget R = code; otherwise code & 0x7f is
 // the largest l_K under this node.
-template <int K, int D, int LT, int J, int B, bool EXPL, class T, class Acc>
+template <int K, int D, int LT, int J, int B, bool EXPL, class T, class St>
 struct Walker {
     using Ln = Lane<D, LT, J, T>;
     static __device__ __forceinline__ void run(const Ln& ln, const int* lim, T pp, uint32_t off,
-                                               int hsum, int rem, State<T, Acc>& st) {
+                                               int hsum, int rem, St& st) {
         int b;
         if constexpr (EXPL) {
             const int code = st.bounds[st.bq++];
@@ -272,8 +334,8 @@ struct Walker {
             const T phi = T(1) - fabs(m);
             const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
             if constexpr (K == 0) st.leaf(pp * phi, o);
-            else Walker<K - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, lim, pp * phi, o, hsum + l,
-                                                                rem - l, st);
+            else Walker<K - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, pp * phi, o, hsum + l,
+                                                               rem - l, st);
             const bool ge = m >= T(0);
             t = 2u * t + (ge ? 1u : 0u);
             m = T(2) * m + (ge ? T(-1) : T(1));
@@ -281,7 +343,7 @@ struct Walker {
     }
 };
 
-template <int D, int LT, int J, int B, int W, bool EXPL, class T, class Acc>
+template <int D, int LT, int J, int B, int W, bool EXPL, bool STG, class T, class Acc>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
 void k_eval(Params<T> prm) {
     int gi;
@@ -318,11 +380,31 @@ void k_eval(Params<T> prm) {
     for (int k = 0; k < D; ++k) lim[k] = G->limits[k];
     // buffer descriptor word 3 for gfx9-family raw (untyped) buffer access
     const T* alpha = prm.alpha ? prm.alpha + G->alpha_off : static_cast<const T*>(G->alpha_ptr);
-    State<T, Acc> st{__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(alpha), short(0),
-                                                       int(G->N * int64_t(sizeof(T))),
-                                                       0x00020000),
-                     as_const(G->leaf_base), as_const(G->bounds), 0, 0, Acc()};
-    Walker<D - 1, D, LT, J, B, EXPL, T, Acc>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
+    using St = State<T, Acc, STG>;
+    St st{};
+    st.alpha = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(alpha), short(0),
+                                                 int(G->N * int64_t(sizeof(T))), 0x00020000);
+    st.leaf_base = as_const(G->leaf_base);
+    st.bounds = as_const(G->bounds);
+    st.q = 0;
+    st.bq = 0;
+    if constexpr (STG) {
+        __shared__ __attribute__((aligned(16))) uint8_t stage_buf[2 * kStageBytes];
+        const cptr<GridDesc::Stages> tb = &G->stage[sizeof(T) == 8 ? 0 : 1];
+        st.lds = (lds_byte*)stage_buf;
+        st.leaf_lds = as_const(tb->leaf_lds);
+        st.stage_end = as_const(tb->stage_end);
+        st.seg_begin = as_const(tb->seg_begin);
+        st.segs = as_const(tb->segs);
+        st.num_stages = tb->num_stages;
+        st.stage = 0;
+        st.cur = 0;
+        st.issue(0, 0);
+        __syncthreads();
+        st.qend = st.stage_end[0];
+        if (st.num_stages > 1) st.issue(1, uint32_t(kStageBytes));
+    }
+    Walker<D - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
     if (live) prm.out[G->out_off + pc] = st.acc.value();
 }
 
@@ -331,16 +413,19 @@ void k_eval(Params<T> prm) {
 // larger batches keep more gathers in flight per wave, fewer registers more waves.
 struct Variant {
     int J, B, W;
+    bool staged;
 };
-constexpr Variant kVariants[] = {{3, 8, 3}, {2, 8, 3}, {3, 4, 4}};
+constexpr Variant kVariants[] = {{3, 8, 3, false}, {2, 8, 3, false}, {3, 4, 4, false},
+                                 {kStageJ, kStageBatch, 3, true}};
+static_assert(kStagedVariant == 3 && kVariants[kStagedVariant].staged, "staged variant index");
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // tabled dimensions the explicit-set kernels use (their node-code stream depends on it)
 constexpr int kExplicitJ = kExplicitTabDims;
 
-template <int D, int LT, int J, int B, int W, bool EXPL, class T, class Acc>
+template <int D, int LT, int J, int B, int W, bool EXPL, class T, class Acc, bool STG = false>
 hipError_t launch_one(const Params<T>& prm, int64_t blocks, int block, hipStream_t s) {
     constexpr int Jd = J < D ? J : D;
-    hipLaunchKernelGGL((k_eval<D, LT, Jd, B, W, EXPL, T, Acc>), dim3(unsigned(blocks)),
+    hipLaunchKernelGGL((k_eval<D, LT, Jd, B, W, EXPL, STG, T, Acc>), dim3(unsigned(blocks)),
                        dim3(block), 0, s, prm);
     return hipGetLastError();
 }
@@ -355,11 +440,18 @@ hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int64_t blo
 {
     if (lt > 8) return launch_one<D, 16, 0, 8, 0, EXPL, T, Acc>(prm, blocks, block, s);
     if constexpr (EXPL) {
+        static_assert(kExplicitJ == kStageJ, "explicit walk and stage plan table alike");
+        if (variant == kStagedVariant)
+            return launch_one<D, 8, kExplicitJ, kStageBatch, 3, true, T, Acc, true>(prm, blocks,
+                                                                                 block, s);
         return launch_one<D, 8, kExplicitJ, 8, 3, true, T, Acc>(prm, blocks, block, s);
     } else {
         switch (variant) {
             case 1: return launch_one<D, 8, 2, 8, 3, false, T, Acc>(prm, blocks, block, s);
             case 2: return launch_one<D, 8, 3, 4, 4, false, T, Acc>(prm, blocks, block, s);
+            case kStagedVariant:
+                return launch_one<D, 8, kStageJ, kStageBatch, 3, false, T, Acc, true>(
+                    prm, blocks, block, s);
             default: return launch_one<D, 8, 3, 8, 3, false, T, Acc>(prm, blocks, block, s);
         }
     }
@@ -414,7 +506,9 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
         set_error("evaluate: surpluses beyond 2 GiB");
         return kRange;
     }
-    const int64_t blocks = (P + tu.block - 1) / tu.block;
+    const int v = effective_variant(g, tu.variant, int(sizeof(T)));
+    const int block = kVariants[v].staged ? kThreads : tu.block;
+    const int64_t blocks = (P + block - 1) / block;
     if (blocks > int64_t(0x7fffffff)) {
         set_error("evaluate: too many points for one launch");
         return kRange;
@@ -428,9 +522,9 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
     prm.perm = perm;
     prm.P = P;
     const hipError_t e = g.explicit_set
-        ? launch_d<true, T, Acc>(g.d, 0, table_levels(g, 0), prm, blocks, tu.block, s)
-        : launch_d<false, T, Acc>(g.d, tu.variant, table_levels(g, tu.variant), prm, blocks,
-                                  tu.block, s);
+        ? launch_d<true, T, Acc>(g.d, v == kStagedVariant ? v : 0, table_levels(g, 0), prm,
+                                 blocks, block, s)
+        : launch_d<false, T, Acc>(g.d, v, table_levels(g, v), prm, blocks, block, s);
     if (e != hipSuccess) {
         set_error(std::string("k_eval launch: ") + hipGetErrorString(e));
         return kHip;

@@ -283,6 +283,8 @@ struct DfsWalk {
     int J;
     std::vector<uint8_t>* codes;
     std::vector<int32_t>* leaves;
+    std::vector<int32_t>* batches = nullptr;  // gather batch sizes, walk order
+    int B = 1;
     int lv[kMaxDims] = {};
     bool ok = true;
 
@@ -334,26 +336,47 @@ struct DfsWalk {
     void leaf() {
         int64_t id = subspace_id(g, lv);
         if (id < 0) { ok = false; return; }
-        leaves->push_back(g.sub_base[id]);
+        if (leaves) leaves->push_back(g.sub_base[id]);
+    }
+
+    // budget R of the static block below a depth-J node, or -1: for an explicit set
+    // the subtree must be exactly { l_0 + .. + l_{J-1} <= R }; for a restricted-regular
+    // set the remaining budget must fit every tabled dimension's limit (the kernel's
+    // test in its walk)
+    int block_budget(int k, int rem) {
+        if (g.explicit_set) return full_block_budget(k, rem);
+        for (int j = 0; j < J; ++j)
+            if (rem > g.limits[j]) return -1;
+        return rem <= 7 ? rem : -1;
     }
 
     void go(int k, int rem) {
         int b = bound_at(k, rem);
-        if (codes && J > 0 && k == J - 1) {
-            int R = full_block_budget(k, rem);
+        if (J > 0 && k == J - 1 && (codes || batches)) {
+            int R = block_budget(k, rem);
             if (R >= 0) {
-                codes->push_back(static_cast<uint8_t>(R));
+                if (codes) codes->push_back(static_cast<uint8_t>(R));
+                if (batches) {
+                    int64_t C = 1;
+                    for (int i = 1; i <= J; ++i) C = C * (R + i) / i;   // C(R + J, J)
+                    for (int64_t c = 0; c < C; c += B)
+                        batches->push_back(int32_t(std::min<int64_t>(B, C - c)));
+                }
                 block(k, R);
                 return;
             }
-            codes->push_back(static_cast<uint8_t>(0x80 | b));
+            if (codes) codes->push_back(static_cast<uint8_t>(0x80 | b));
         } else if (codes) {
             codes->push_back(static_cast<uint8_t>(b));
         }
         for (int l = 0; l <= b; ++l) {
             lv[k] = l;
-            if (k == 0) leaf();
-            else go(k - 1, rem - l);
+            if (k == 0) {
+                leaf();
+                if (batches) batches->push_back(1);
+            } else {
+                go(k - 1, rem - l);
+            }
         }
         lv[k] = 0;
     }
@@ -378,6 +401,55 @@ int explicit_tab_dims(const Grid& g) {
     return J;
 }
 
+namespace {
+
+// Pack the walk's leaves into LDS stages for one element size (see StagePlan).
+void build_stage_plan(const Grid& g, const std::vector<int32_t>& batches, int elem,
+                      StagePlan& p) {
+    p = StagePlan();
+    // block size of each subspace, by base offset
+    std::unordered_map<int32_t, int64_t> size_of;
+    size_of.reserve(size_t(g.S) * 2);
+    for (int64_t id = 0; id < g.S; ++id) {
+        const int64_t end = id + 1 < g.S ? g.sub_base[id + 1] : g.N;
+        size_of[g.sub_base[id]] = end - g.sub_base[id];
+    }
+    p.leaf_lds.assign(size_t(g.S), 0);
+    p.seg_begin.push_back(0);
+    int64_t q = 0;
+    int32_t used = 0;
+    for (int32_t nb : batches) {
+        int32_t bytes = 0;
+        for (int32_t c = 0; c < nb; ++c) {
+            const int64_t b = size_of[g.dfs_base[q + c]] * elem;
+            if (b > kStageMaxBlock) return;                     // not stageable
+            bytes += int32_t((b + 15) & ~int64_t(15));
+        }
+        if (bytes > kStageBytes) return;
+        if (used + bytes > kStageBytes) {                       // close the stage
+            p.stage_end.push_back(int32_t(q));
+            p.seg_begin.push_back(int32_t(p.segs.size() / 2));
+            used = 0;
+        }
+        for (int32_t c = 0; c < nb; ++c, ++q) {
+            const int32_t base = g.dfs_base[q];
+            const int32_t b = int32_t(size_of[base] * elem);
+            p.leaf_lds[q] = used;
+            for (int32_t o = 0; o < b; o += kStageSegBytes) {
+                const int32_t piece = std::min(kStageSegBytes, b - o);
+                p.segs.push_back(int32_t(int64_t(base) * elem + o));
+                p.segs.push_back((used + o) | (((piece + 15) / 16) << 16));
+            }
+            used += (b + 15) & ~15;
+        }
+    }
+    p.stage_end.push_back(int32_t(q));
+    p.seg_begin.push_back(int32_t(p.segs.size() / 2));
+    p.ok = q == g.S && g.N * elem <= int64_t(0x7fffffff);
+}
+
+}  // namespace
+
 int build_launch_tables(Grid& g, int hier_tile_values) {
     const int d = g.d;
     // evaluation walk
@@ -390,6 +462,28 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
     if (!w.ok || int64_t(g.dfs_base.size()) != g.S) {
         set_error("build_launch_tables: depth-first walk does not cover the level set");
         return kInvalid;
+    }
+    // staged walk: batches of the walk with kStageJ tabled dimensions (levels of the
+    // tabled dimensions below 8), packed into LDS stages
+    g.stage[0] = StagePlan();
+    g.stage[1] = StagePlan();
+    {
+        const int J = g.explicit_set ? explicit_tab_dims(g) : std::min(d, kStageJ);
+        bool tabled = J > 0;
+        for (int j = 0; j < J; ++j) tabled = tabled && g.max_level[j] <= 7 && g.limits[j] <= 7;
+        if (tabled) {
+            std::vector<int32_t> batches;
+            DfsWalk bw{g, J, nullptr, nullptr};
+            bw.batches = &batches;
+            bw.B = kStageBatch;
+            bw.go(d - 1, g.n - 1);
+            int64_t total = 0;
+            for (int32_t b : batches) total += b;
+            if (bw.ok && total == g.S) {
+                build_stage_plan(g, batches, 8, g.stage[0]);
+                build_stage_plan(g, batches, 4, g.stage[1]);
+            }
+        }
     }
     // hierarchization: pole families per dimension
     g.hier.assign(d, HierTables());

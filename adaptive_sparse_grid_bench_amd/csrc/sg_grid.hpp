@@ -29,6 +29,17 @@ constexpr int kMaxLevel = 15;       // per-dimension level cap of the kernels' t
 constexpr int kMaxLevelSum = 26;    // keeps 2^s * sizeof(double) inside 32-bit offsets
 constexpr int kExplicitTabDims = 3; // register-tabled dims of the explicit-set walk
 
+// Staged evaluation (surplus blocks copied into LDS by the workgroup, then read by each
+// lane): two stage buffers of kStageBytes per workgroup, blocks of at most
+// kStageMaxBlock bytes, copied in segments of at most kStageSegBytes (one wave
+// instruction of 16 bytes per lane).  The staged walk tables kStageJ dimensions and
+// issues its reads in batches of kStageBatch leaves.
+constexpr int kStageBytes = 16384;
+constexpr int kStageMaxBlock = 4096;
+constexpr int kStageSegBytes = 1024;
+constexpr int kStageJ = 3;
+constexpr int kStageBatch = 8;
+
 struct HierTables {
     // One entry per tile / family / block of the pole-family decomposition of one
     // dimension.  A family is the set of subspaces that differ only in l_k; its blocks
@@ -38,6 +49,18 @@ struct HierTables {
     std::vector<int32_t> blocks;    // base offset of each block
     int max_family_values = 0;
     int max_families_per_tile = 0;
+};
+
+// How the staged walk copies one grid's surplus blocks into LDS, for one element size.
+// Leaves (subspaces in walk order) are packed into stages of at most kStageBytes, each
+// leaf at a 16-byte aligned offset; a stage ends only where a gather batch of the walk
+// ends, so the kernel checks for a stage change once per batch.
+struct StagePlan {
+    bool ok = false;                  // every block fits: the staged walk may run
+    std::vector<int32_t> leaf_lds;    // [S] byte offset of each leaf in its stage
+    std::vector<int32_t> stage_end;   // [K] first leaf after stage k
+    std::vector<int32_t> seg_begin;   // [K+1] first copy segment of stage k
+    std::vector<int32_t> segs;        // [nseg][2]: source byte offset, lds offset | lanes << 16
 };
 
 struct Grid {
@@ -59,6 +82,7 @@ struct Grid {
     std::vector<int32_t> dfs_base;    // [S] base offset per leaf, in walk order
     std::vector<uint8_t> dfs_bound;   // explicit sets: per walk node, the largest l_k
     std::vector<HierTables> hier;     // [d]
+    StagePlan stage[2];               // [0]: 8-byte surpluses, [1]: 4-byte
 };
 
 // status codes shared with the C ABI

@@ -544,6 +544,25 @@ int upload_tables(const Grid& g, DeviceTables& t) {
     desc.N = g.N;
     desc.n = g.n;
     for (int j = 0; j < g.d; ++j) desc.limits[j] = g.limits[j];
+    for (int e = 0; e < 2; ++e) {
+        const StagePlan& p = g.stage[e];
+        if (!p.ok) continue;
+        // one allocation: leaf_lds | stage_end | seg_begin | segs
+        std::vector<int32_t> all;
+        all.reserve(p.leaf_lds.size() + p.stage_end.size() + p.seg_begin.size() + p.segs.size());
+        all.insert(all.end(), p.leaf_lds.begin(), p.leaf_lds.end());
+        all.insert(all.end(), p.stage_end.begin(), p.stage_end.end());
+        all.insert(all.end(), p.seg_begin.begin(), p.seg_begin.end());
+        all.insert(all.end(), p.segs.begin(), p.segs.end());
+        if ((st = upload(t.stage_tabs[e], all.data(), all.size())) != kOk) return st;
+        eval::GridDesc::Stages& sd = desc.stage[e];
+        sd.leaf_lds = t.stage_tabs[e];
+        sd.stage_end = sd.leaf_lds + p.leaf_lds.size();
+        sd.seg_begin = sd.stage_end + p.stage_end.size();
+        sd.segs = sd.seg_begin + p.seg_begin.size();
+        sd.num_stages = int32_t(p.stage_end.size());
+    }
+    t.host_desc = desc;
     eval::GridDesc* ddesc = nullptr;
     if ((st = upload(ddesc, &desc, 1)) != kOk) return st;
     t.desc = ddesc;
@@ -573,6 +592,8 @@ void free_tables(DeviceTables& t) {
     free_dev(t.sub_base);
     free_dev(t.dfs_base);
     free_dev(t.dfs_bound);
+    free_dev(t.stage_tabs[0]);
+    free_dev(t.stage_tabs[1]);
     if (t.desc) (void)hipFree(t.desc);
     t.desc = nullptr;
     for (auto& dm : t.dims) {

@@ -25,6 +25,16 @@ struct GridDesc {
     int64_t N;                  // grid points (the buffer range of its surpluses)
     int32_t n;                  // level sums 0..n-1
     int32_t limits[kMaxDims];
+    // LDS stage tables of the staged walk ([0]: 8-byte surpluses, [1]: 4-byte), see
+    // StagePlan; num_stages = 0 when the grid cannot be staged
+    struct Stages {
+        const int32_t* leaf_lds;
+        const int32_t* stage_end;
+        const int32_t* seg_begin;
+        const int32_t* segs;
+        int32_t num_stages;
+        int32_t pad;
+    } stage[2];
 };
 
 // A run of at most one workgroup's points of one grid.
@@ -47,7 +57,9 @@ struct DeviceTables {
     int32_t* sub_base = nullptr;    // [S]
     int32_t* dfs_base = nullptr;    // [S]
     uint8_t* dfs_bound = nullptr;   // explicit sets: walk-node bounds
+    int32_t* stage_tabs[2] = {};    // StagePlan arrays per element size, one allocation each
     void* desc = nullptr;           // the grid's evaluation descriptor (eval::GridDesc)
+    eval::GridDesc host_desc{};     // host copy: tables set, launch offsets zero
     struct Dim {
         int32_t* tiles = nullptr;
         int32_t* fams = nullptr;
@@ -78,12 +90,21 @@ int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipSt
 
 struct EvalTuning {
     int block = 256;            // threads per workgroup
-    int variant = 0;            // register-table walk variant (sg_eval.hpp kVariants)
+    int variant = 0;            // walk variant (sg_eval.hpp kVariants)
     int bin_min_points = 4096;  // spatially bin the points of launches at least this big
 };
 
 // default variants by dimension and precision (measured at d = 10, 12, 16, 20)
 inline int default_variant(int d, bool f32) { return f32 ? 2 : (d >= 16 ? 1 : 0); }
+
+// the walk variant that reads surpluses from LDS stages
+constexpr int kStagedVariant = 3;
+// the variant a launch runs: the staged walk falls back to variant 0 for a grid whose
+// blocks cannot be staged in this element size
+inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
+    if (variant == kStagedVariant && !g.stage[elem_bytes == 8 ? 0 : 1].ok) return 0;
+    return variant;
+}
 
 // perm: lane order of the points (from sort_points) or null for the given order
 int launch_evaluate_f64(const Grid& g, const DeviceTables& t, const double* alpha,

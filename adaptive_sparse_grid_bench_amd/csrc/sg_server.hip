@@ -42,6 +42,7 @@ struct Server::Impl {
         const Grid* g;
         const DeviceTables* t;
         const void* alpha;      // device surpluses in the server's precision
+        int variant;            // walk variant that runs for this grid
     };
     struct Request {
         int64_t id;
@@ -190,7 +191,7 @@ struct Server::Impl {
         std::vector<Request*> rs;
         for (int64_t id : batch) rs.push_back(&reqs[id]);
         lk.unlock();
-        std::map<std::tuple<int, int, int>, std::vector<eval::WorkItem>> groups;
+        std::map<std::tuple<int, int, int, int>, std::vector<eval::WorkItem>> groups;
         int64_t coff = 0, ooff = 0;
         std::vector<int64_t> out_off(rs.size());
         for (size_t i = 0; i < rs.size(); ++i) {
@@ -200,17 +201,13 @@ struct Server::Impl {
             std::memcpy(static_cast<char*>(s.h_coords) + coff * elem, r.points,
                         size_t(r.P) * g.d * elem);
             eval::GridDesc& D = s.h_desc[i];
+            D = ge.t->host_desc;                 // walk and stage tables
             D.alpha_off = 0;
             D.alpha_ptr = ge.alpha;
             D.coord_off = coff;
             D.out_off = ooff;
-            D.leaf_base = ge.t->dfs_base;
-            D.bounds = ge.t->dfs_bound;
-            D.N = g.N;
-            D.n = g.n;
-            for (int j = 0; j < g.d; ++j) D.limits[j] = g.limits[j];
             auto& items = groups[std::make_tuple(g.explicit_set ? 1 : 0, g.d,
-                                                 eval::table_levels(g, 0))];
+                                                 eval::table_levels(g, 0), ge.variant)];
             for (int64_t f = 0; f < r.P; f += eval::kThreads)
                 items.push_back({int32_t(i), int32_t(f),
                                  int32_t(std::min<int64_t>(eval::kThreads, r.P - f)), 0});
@@ -219,12 +216,12 @@ struct Server::Impl {
             ooff += r.P;
         }
         int64_t nitems = 0;
-        std::vector<std::tuple<int, int, int, int64_t, int64_t>> launches_v;
+        std::vector<std::tuple<int, int, int, int, int64_t, int64_t>> launches_v;
         for (auto& kv : groups) {
             std::memcpy(s.h_work + nitems, kv.second.data(), kv.second.size() * sizeof(eval::WorkItem));
             launches_v.push_back(std::make_tuple(std::get<0>(kv.first), std::get<1>(kv.first),
-                                                 std::get<2>(kv.first), nitems,
-                                                 int64_t(kv.second.size())));
+                                                 std::get<2>(kv.first), std::get<3>(kv.first),
+                                                 nitems, int64_t(kv.second.size())));
             nitems += int64_t(kv.second.size());
         }
         hipError_t e = hipMemcpyAsync(s.d_coords, s.h_coords, size_t(coff) * elem,
@@ -235,9 +232,9 @@ struct Server::Impl {
         if (e == hipSuccess)
             e = hipMemcpyAsync(s.d_work, s.h_work, size_t(nitems) * sizeof(eval::WorkItem),
                                hipMemcpyHostToDevice, s.stream);
-        for (auto& [expl, d, lt, first, count] : launches_v) {
+        for (auto& [expl, d, lt, variant, first, count] : launches_v) {
             if (e != hipSuccess) break;
-            e = launch_group(expl != 0, d, lt, first, count, s);
+            e = launch_group(expl != 0, d, lt, variant, first, count, s);
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(s.h_out, s.d_out, size_t(ooff) * elem, hipMemcpyDeviceToHost,
@@ -260,7 +257,8 @@ struct Server::Impl {
         return true;
     }
 
-    hipError_t launch_group(bool expl, int d, int lt, int64_t first, int64_t count, Slot& s) {
+    hipError_t launch_group(bool expl, int d, int lt, int variant, int64_t first, int64_t count,
+                            Slot& s) {
         if (elem == 8) {
             eval::Params<double> prm{};
             prm.alpha = nullptr;                       // descriptors carry the addresses
@@ -268,11 +266,10 @@ struct Server::Impl {
             prm.out = static_cast<double*>(s.d_out);
             prm.grids = s.d_desc;
             prm.work = s.d_work + first;
-            return expl ? eval::launch_d<true, double, eval::AccF64>(d, 0, lt, prm, count,
+            return expl ? eval::launch_d<true, double, eval::AccF64>(d, variant, lt, prm, count,
                                                                      eval::kThreads, s.stream)
                         : eval::launch_d<false, double, eval::AccF64>(
-                              d, default_variant(d, false), lt, prm, count, eval::kThreads,
-                              s.stream);
+                              d, variant, lt, prm, count, eval::kThreads, s.stream);
         }
         eval::Params<float> prm{};
         prm.alpha = nullptr;
@@ -280,11 +277,10 @@ struct Server::Impl {
         prm.out = static_cast<float*>(s.d_out);
         prm.grids = s.d_desc;
         prm.work = s.d_work + first;
-        return expl ? eval::launch_d<true, float, eval::AccKahan>(d, 0, lt, prm, count,
+        return expl ? eval::launch_d<true, float, eval::AccKahan>(d, variant, lt, prm, count,
                                                                   eval::kThreads, s.stream)
                     : eval::launch_d<false, float, eval::AccKahan>(
-                          d, default_variant(d, true), lt, prm, count, eval::kThreads,
-                          s.stream);
+                          d, variant, lt, prm, count, eval::kThreads, s.stream);
     }
 
     void loop() {
@@ -333,13 +329,14 @@ int Server::start(const ServerConfig& cfg) {
     return p->init();
 }
 
-int Server::add_grid(const Grid* g, const DeviceTables* t, const void* alpha, int* id) {
+int Server::add_grid(const Grid* g, const DeviceTables* t, const void* alpha, int variant,
+                     int* id) {
     std::lock_guard<std::mutex> lk(p->mu);
     if (t->device != p->device) {
         set_error("server: grid lives on another device");
         return kInvalid;
     }
-    p->grids.push_back({g, t, alpha});
+    p->grids.push_back({g, t, alpha, effective_variant(*g, variant, int(p->elem))});
     *id = int(p->grids.size()) - 1;
     return kOk;
 }

@@ -21,7 +21,8 @@ class Server {
     Server();
     ~Server();
     int start(const ServerConfig& cfg);
-    int add_grid(const Grid* g, const DeviceTables* t, const void* alpha, int* id);
+    // variant: the grid's walk variant for the server's precision
+    int add_grid(const Grid* g, const DeviceTables* t, const void* alpha, int variant, int* id);
     int submit(int grid, const void* points, int64_t P, void* out, int64_t* id);
     int wait(int64_t id, double* latency_s);
     int poll(int64_t id, int* done);

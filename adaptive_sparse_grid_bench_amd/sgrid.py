@@ -54,7 +54,8 @@ class Tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-EVAL_VARIANTS = 3
+EVAL_VARIANTS = 4
+STAGED_VARIANT = 3          # surplus blocks staged in LDS by the workgroup
 
 
 class ServerConfig(C.Structure):

@@ -78,14 +78,19 @@ int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *bas
  * chosen from the grid's shape); sg_grid_set_tuning replaces them (not concurrently
  * with other calls on the same grid).  The autotuner (package module autotune.py)
  * times the candidates on the GPU and stores its choice per grid shape.
- *   eval_block        threads per workgroup of the evaluate kernels: 64, 128 or 256;
- *   eval_variant_f64  register-table walk variant of the FP64 evaluate,
- *   eval_variant_f32  and of the FP32 one: 0..SG_EVAL_VARIANTS-1 (tabled dimensions,
- *                     gathers per batch and waves per SIMD; see the design notes);
+ *   eval_block        threads per workgroup of the evaluate kernels: 64, 128 or 256
+ *                     (the staged variant always runs 256);
+ *   eval_variant_f64  walk variant of the FP64 evaluate,
+ *   eval_variant_f32  and of the FP32 one: 0..SG_EVAL_VARIANTS-1.  0..2 gather each
+ *                     coefficient per lane from HBM/cache (they differ in tabled
+ *                     dimensions, gathers per batch and waves per SIMD); 3 is the staged
+ *                     walk, where the workgroup copies surplus blocks into LDS and the
+ *                     lanes read them there.  A grid with blocks over 4 KiB (or tabled
+ *                     dimensions above level 7) runs variant 0 instead of 3;
  *   bin_min_points    evaluate launches with at least this many points bin the points
  *                     spatially first (<= 0: never);
  *   hier_tile_values  target values per hierarchization workgroup tile (>= 64). */
-#define SG_EVAL_VARIANTS 3
+#define SG_EVAL_VARIANTS 4
 typedef struct sg_tuning {
     int eval_block;
     int eval_variant_f64;

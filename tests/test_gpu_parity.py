@@ -274,7 +274,7 @@ def test_evaluate_f32c_error_bound(sg, torch_gpu):
 
 # ------------------------------------------------ variants, binning, explicit sets --
 
-@pytest.mark.parametrize("variant", range(3))
+@pytest.mark.parametrize("variant", range(4))
 @pytest.mark.parametrize("binning", [-1, 1])
 @pytest.mark.parametrize("d,n,P", [(10, 8, 3000), (5, 6, 5000), (2, 8, 4000)])
 def test_evaluate_variants_and_binning(sg, torch_gpu, variant, binning, d, n, P):

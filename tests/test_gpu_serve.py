@@ -46,11 +46,16 @@ def alone(torch, g, alpha, pts):
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32c"])
-def test_alone_equals_in_the_mix(sg, torch_gpu, resident, precision):
+@pytest.mark.parametrize("variant", [None, 3], ids=["default", "staged"])
+def test_alone_equals_in_the_mix(sg, torch_gpu, resident, precision, variant):
     torch = torch_gpu
     dt = np.float64 if precision == "f64" else np.float32
     srv = sg.Server(precision, slots=3, max_batch_points=1 << 14, max_requests=64)
     alphas = [a.to(torch.float64 if dt == np.float64 else torch.float32) for _, a in resident]
+    saved = [g.get_tuning() for g, _ in resident]
+    if variant is not None:                 # the server takes each grid's variant at add
+        for g, _ in resident:
+            g.set_tuning(eval_variant_f64=variant, eval_variant_f32=variant)
     ids = [srv.add_grid(g, a) for (g, _), a in zip(resident, alphas)]
     rng = np.random.default_rng(7)
     reqs = []
@@ -81,6 +86,8 @@ def test_alone_equals_in_the_mix(sg, torch_gpu, resident, precision):
         ref = alone(torch, g, alphas[gi], pts)
         assert np.array_equal(out, ref)
     srv.close()
+    for (g, _), t in zip(resident, saved):
+        g.set_tuning(**t)
 
 
 def test_budget_admission_and_errors(sg, torch_gpu, resident):

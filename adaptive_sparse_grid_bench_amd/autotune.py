@@ -50,9 +50,10 @@ def _store_cache(path: str, cache: dict) -> None:
 
 
 def apply_cached(g: "sgrid.Grid", cache_path: str = DEFAULT_CACHE) -> bool:
-    """Apply a stored choice for this grid's shape; False if there is none."""
+    """Apply a stored choice for this grid's shape; False if there is none (or it was
+    timed on another version of the kernels)."""
     entry = _load_cache(cache_path).get(shape_key(g))
-    if not entry:
+    if not entry or entry.get("library_version") != sgrid.lib.sg_version():
         return False
     g.set_tuning(**entry["tuning"])
     return True
@@ -96,7 +97,8 @@ def tune(g: "sgrid.Grid", points: int = 1 << 17, reps: int = 3, precisions=("f64
         for variant in range(sgrid.EVAL_VARIANTS):
             # the staged walk always runs 256-thread workgroups
             for block in ((256,) if variant == sgrid.STAGED_VARIANT else EVAL_BLOCKS):
-                for binning in (-1, 4096):
+                # the staged walk never bins
+                for binning in ((-1,) if variant == sgrid.STAGED_VARIANT else (-1, 4096)):
                     g.set_tuning(**{**base, key: variant, "eval_block": block,
                                     "bin_min_points": binning})
                     ms = _time(torch, lambda: g.evaluate(a, x, o), reps)
@@ -122,7 +124,8 @@ def tune(g: "sgrid.Grid", points: int = 1 << 17, reps: int = 3, precisions=("f64
     best["hier_tile_values"] = hcand[0][1]
     g.set_tuning(**best)
     record = {"tuning": best, "timings_ms": timings, "points": points,
-              "tuned_in_s": round(time.time() - t_start, 2)}
+              "tuned_in_s": round(time.time() - t_start, 2),
+              "library_version": sgrid.lib.sg_version()}
     if cache_path:
         cache = _load_cache(cache_path)
         cache[shape_key(g)] = record

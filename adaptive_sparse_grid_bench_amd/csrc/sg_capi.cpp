@@ -87,8 +87,12 @@ int sort_one(const float* x, int64_t P, int d, int kb, void* ws, uint32_t* perm,
 template <class T>
 int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, void* stream) {
     hipStream_t s = as_stream(stream);
-    // binning pays when the walk per point is long: skip it for small grids
-    if (P < h->tune.bin_min_points || h->tune.bin_min_points <= 0 || h->g.S < 1024)
+    // binning pays when the walk per point is long: skip it for small grids, and for the
+    // staged walk, whose LDS reads gain nothing from neighbouring lanes sharing cells
+    const int v = sg::effective_variant(h->g, sizeof(T) == 8 ? h->tune.variant : h->tune32.variant,
+                                        int(sizeof(T)));
+    if (P < h->tune.bin_min_points || h->tune.bin_min_points <= 0 || h->g.S < 1024 ||
+        v == sg::kStagedVariant)
         return eval_one(h, a, x, P, o, nullptr, s);
     const int kb = sg::key_bits_for(P, h->g.d);
     const size_t ws_bytes = sg::sort_workspace_bytes(P, kb);
@@ -128,7 +132,7 @@ extern "C" {
 
 const char* sg_last_error(void) { return sg::last_error().c_str(); }
 const char* sg_status_string(int status) { return sg::status_name(status); }
-int sg_version(void) { return 100; }
+int sg_version(void) { return 110; }
 
 int sg_grid_create(int num_dims, int num_levels, const int* limits, sg_grid** out) {
     if (!out) return fail(sg::kInvalid, "sg_grid_create: out is NULL");
@@ -345,6 +349,90 @@ int sg_uniform_points_f32(long long count, unsigned long long seed, float* d_out
                           void* stream) {
     if (count < 0 || (count > 0 && !d_out)) return fail(sg::kInvalid, "sg_uniform_points_f32: bad argument");
     return sg::launch_uniform_points_f32(count, seed, d_out, as_stream(stream));
+}
+
+namespace {
+int sample_params(const sg_grid* grid, int function, const double* params, long long num_params,
+                  sg::SampleParams& sp) {
+    const int d = grid->g.d;
+    sp.fn = function;
+    if (num_params < 0 || (num_params > 0 && !params))
+        return fail(sg::kInvalid, "sg_sample: bad parameter array");
+    switch (function) {
+        case SG_FN_REFERENCE:
+        case SG_FN_PARABOLA:
+            if (num_params != 0) return fail(sg::kInvalid, "sg_sample: function takes no parameters");
+            return sg::kOk;
+        case SG_FN_GAUSS:
+            if (num_params != 0 && num_params != d)
+                return fail(sg::kInvalid, "sg_sample: Gaussian takes 0 or d centres");
+            for (int k = 0; k < d; ++k) sp.centre[0][k] = num_params ? params[k] : 0.5;
+            return sg::kOk;
+        case SG_FN_GAUSSIAN_SUM: {
+            if (num_params < 1) return fail(sg::kInvalid, "sg_sample: Gaussian sum needs its count");
+            const double tc = params[0];
+            const int terms = int(tc);
+            if (double(terms) != tc || terms < 1 || terms > sg::kMaxSampleTerms ||
+                num_params != 1 + 2LL * terms * d)
+                return fail(sg::kInvalid, "sg_sample: Gaussian sum takes [count, centres, widths]");
+            sp.terms = terms;
+            for (int t = 0; t < terms; ++t)
+                for (int k = 0; k < d; ++k) {
+                    sp.centre[t][k] = params[1 + t * d + k];
+                    sp.width[t][k] = params[1 + terms * d + t * d + k];
+                    if (!(sp.width[t][k] > 0.0))
+                        return fail(sg::kInvalid, "sg_sample: Gaussian widths must be positive");
+                }
+            return sg::kOk;
+        }
+        case SG_FN_SIN_PRODUCT:
+            if (num_params > 1) return fail(sg::kInvalid, "sg_sample: sine product takes 0 or 1 factor");
+            sp.scale = num_params ? params[0] : 1.0;
+            return sg::kOk;
+        default:
+            return fail(sg::kInvalid, "sg_sample: unknown function");
+    }
+}
+}  // namespace
+
+int sg_sample_f64(const sg_grid* grid, int function, const double* params, long long num_params,
+                  double* d_values, void* stream) {
+    if (!grid || !d_values) return fail(sg::kInvalid, "sg_sample_f64: NULL argument");
+    sg::SampleParams sp;
+    int st = sample_params(grid, function, params, num_params, sp);
+    if (st != sg::kOk) return st;
+    if ((st = check_device(grid)) != sg::kOk) return st;
+    return sg::launch_sample_f64(grid->g, grid->t, sp, d_values, as_stream(stream));
+}
+
+int sg_sample_f32(const sg_grid* grid, int function, const double* params, long long num_params,
+                  float* d_values, void* stream) {
+    if (!grid || !d_values) return fail(sg::kInvalid, "sg_sample_f32: NULL argument");
+    sg::SampleParams sp;
+    int st = sample_params(grid, function, params, num_params, sp);
+    if (st != sg::kOk) return st;
+    if ((st = check_device(grid)) != sg::kOk) return st;
+    return sg::launch_sample_f32(grid->g, grid->t, sp, d_values, as_stream(stream));
+}
+
+int sg_gather_probe_f64(const sg_grid* grid, const double* d_alpha, long long num_points,
+                        unsigned long long seed, double* d_out, void* stream) {
+    if (!grid || num_points < 0 || (num_points > 0 && (!d_alpha || !d_out)))
+        return fail(sg::kInvalid, "sg_gather_probe_f64: bad argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_gather_probe_f64(grid->g, grid->t, d_alpha, num_points, uint32_t(seed),
+                                       d_out, as_stream(stream));
+}
+
+int sg_gather_probe_f32(const sg_grid* grid, const float* d_alpha, long long num_points,
+                        unsigned long long seed, float* d_out, void* stream) {
+    if (!grid || num_points < 0 || (num_points > 0 && (!d_alpha || !d_out)))
+        return fail(sg::kInvalid, "sg_gather_probe_f32: bad argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_gather_probe_f32(grid->g, grid->t, d_alpha, num_points, uint32_t(seed),
+                                       d_out, as_stream(stream));
 }
 
 int sg_convert_f64_to_f32(const double* d_in, float* d_out, long long count, void* stream) {

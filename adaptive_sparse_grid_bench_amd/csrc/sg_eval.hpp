@@ -16,11 +16,13 @@
 //        allocation targets (0 = compiler's choice);
 //   T    coordinate / surplus type;  Acc  accumulator (FP64 FMA or FP32 Kahan);
 //   EXPL explicit (refined-subspace) level set: node bounds come from a byte stream;
-//   STG  staged: the workgroup copies the surplus blocks of the coming leaves into LDS
-//        (buffer loads straight to LDS, 16 bytes per lane, double-buffered stages) and
-//        each lane reads its coefficient there.  Every lane of a workgroup visits the
-//        same leaves in the same order, so one copy serves all its waves; a direct
-//        gather costs one texture-address pass per wave and leaf instead.
+//   STG  staged: the workgroup copies the surplus blocks of the coming static blocks
+//        into LDS (buffer loads straight to LDS, 16 bytes per lane, double-buffered
+//        stages planned on the host) and each lane reads its coefficient there.  Every
+//        lane of a workgroup visits the same leaves in the same order, so one copy
+//        serves all its waves; a direct gather costs one texture-address pass per wave
+//        and leaf instead.  Blocks too large to share a stage, and leaves outside static
+//        blocks, are still gathered from global memory.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -121,6 +123,7 @@ using lds_byte = __attribute__((address_space(3))) uint8_t;
 
 template <class T, class Acc, bool STG>
 struct State {
+    static constexpr bool kStaged = STG;
     __amdgpu_buffer_rsrc_t alpha;
     cptr<int32_t> leaf_base;
     cptr<uint8_t> bounds;
@@ -129,52 +132,56 @@ struct State {
     Acc acc;
     // staged walk (uniform across the workgroup)
     lds_byte* lds;              // two stage buffers of kStageBytes
-    cptr<int32_t> leaf_lds, stage_end, seg_begin, segs;
-    int stage, num_stages, qend;
-    uint32_t cur;               // byte offset of the stage being read (0 or kStageBytes)
+    cptr<int32_t> leaf_lds, stage_first, seg_begin, segs;
+    int stage, num_stages;
+    int next_first;             // first leaf of the next stage
 
-    // copy stage k's blocks into the stage buffer at byte offset buf: the waves take
-    // the segments in turn, each lane moves 16 bytes
-    __device__ __forceinline__ void issue(int k, uint32_t buf) {
+    // copy stage k's blocks into LDS: each wave takes a contiguous quarter of the
+    // stage's segments, reads their descriptors four at a time and moves 16 bytes per
+    // lane (the descriptor table ends in four empty entries)
+    __device__ __forceinline__ void issue(int k) {
         const int s0 = seg_begin[k], s1 = seg_begin[k + 1];
+        constexpr int kWaves = kThreads / 64;      // staged launches use kThreads
         const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-        constexpr int waves = kThreads / 64;       // staged launches use kThreads
+        const int per = (s1 - s0 + kWaves - 1) / kWaves;
+        const int g0 = s0 + wave * per;
+        const int g1 = min(s1, g0 + per);
         const uint32_t lane = threadIdx.x & 63u;
-        for (int g = s0 + wave; g < s1; g += waves) {
-            const int src = segs[2 * g];
-            const int w1 = segs[2 * g + 1];
-            if (int(lane) < (w1 >> 16))
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    alpha, lds + buf + uint32_t(w1 & 0xffff), 16, lane * 16u, src, 0, 0);
+        for (int g = g0; g < g1; g += 4) {
+            int v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = segs[2 * g + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (g + j < g1 && int(lane) < (v[2 * j + 1] >> 16))
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        alpha, lds + uint32_t(v[2 * j + 1] & 0xffff), 16, lane * 16u, v[2 * j],
+                        0, 0);
+            }
         }
     }
-    // all waves are done with the current stage and the next one has landed: switch to
-    // it and start copying the one after into the buffer just released
+    // every wave is done with the current stage and the next one has landed: move on
+    // and start copying the one after into the buffer just released
     __device__ __forceinline__ void advance() {
         __syncthreads();
         ++stage;
-        cur ^= uint32_t(kStageBytes);
-        qend = stage_end[stage];
-        if (stage + 1 < num_stages) issue(stage + 1, cur ^ uint32_t(kStageBytes));
+        next_first = stage_first[stage + 1];
+        if (stage + 1 < num_stages) issue(stage + 1);
     }
-    // a batch of leaves starting at leaf q0 is about to be read
-    __device__ __forceinline__ void batch(int q0) {
-        if constexpr (STG) {
-            acc.pin();
-            if (q0 == qend) advance();
-        }
+    // a staged static block starts at leaf q0
+    __device__ __forceinline__ void block_start(int q0) {
+        acc.pin();
+        if (q0 == next_first) advance();
     }
-    __device__ __forceinline__ T fetch(int qq, uint32_t voff) {
-        if constexpr (STG) {
-            return *reinterpret_cast<const __attribute__((address_space(3))) T*>(
-                lds + (cur + uint32_t(leaf_lds[qq]) + voff));
-        } else {
-            return gather_leaf<T>(alpha, leaf_base[qq], voff);
-        }
+    __device__ __forceinline__ T fetch_lds(int qq, uint32_t voff) const {
+        return *reinterpret_cast<const __attribute__((address_space(3))) T*>(
+            lds + (uint32_t(leaf_lds[qq]) + voff));
+    }
+    __device__ __forceinline__ T fetch_global(int qq, uint32_t voff) const {
+        return gather_leaf<T>(alpha, leaf_base[qq], voff);
     }
     __device__ __forceinline__ void leaf(T w, uint32_t voff) {
-        batch(q);
-        acc.add(w, fetch(q, voff));
+        acc.add(w, fetch_global(q, voff));
         ++q;
     }
 };
@@ -238,12 +245,14 @@ __device__ __forceinline__ T leaf_weight(const Ln& ln, T w) {
     return w;
 }
 
-template <int J, int R, int B, int c, int cend, class Ln, class T, class St>
+template <int J, int R, int B, bool LDS, int c, int cend, class Ln, class T, class St>
 __device__ __forceinline__ void block_issue(T (&a)[B], const Ln& ln, uint32_t off, int hsum,
                                             St& st) {
     if constexpr (c < cend) {
-        a[c % B] = st.fetch(st.q + c, leaf_offset<J, R, c, J - 1>(ln, off, hsum));
-        block_issue<J, R, B, c + 1, cend>(a, ln, off, hsum, st);
+        const uint32_t o = leaf_offset<J, R, c, J - 1>(ln, off, hsum);
+        if constexpr (LDS) a[c % B] = st.fetch_lds(st.q + c, o);
+        else a[c % B] = st.fetch_global(st.q + c, o);
+        block_issue<J, R, B, LDS, c + 1, cend>(a, ln, off, hsum, st);
     }
 }
 
@@ -255,26 +264,37 @@ __device__ __forceinline__ void block_consume(const T (&a)[B], const Ln& ln, T p
     }
 }
 
-template <int J, int R, int B, int c0, class Ln, class T, class St>
+template <int J, int R, int B, bool LDS, int c0, class Ln, class T, class St>
 __device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, int hsum,
                                               St& st) {
     constexpr int C = Simplex<J, R>::C;
     if constexpr (c0 < C) {
         constexpr int cend = (c0 + B < C) ? c0 + B : C;
         T a[B];
-        st.batch(st.q + c0);
-        block_issue<J, R, B, c0, cend>(a, ln, off, hsum, st);
+        block_issue<J, R, B, LDS, c0, cend>(a, ln, off, hsum, st);
         block_consume<J, R, B, c0, cend>(a, ln, pp, st.acc);
-        block_batches<J, R, B, cend>(ln, pp, off, hsum, st);
+        block_batches<J, R, B, LDS, cend>(ln, pp, off, hsum, st);
     }
 }
 
 // All C(R+J, J) leaves below a depth-J node whose remaining level budget R fits inside
-// the J tabled dimensions.
+// the J tabled dimensions.  The staged walk reads a block from LDS when the host packed
+// it into a stage (the same rule, stage_whole, decides on both sides), else from global
+// memory.
 template <int J, int R, int B, class Ln, class T, class St>
 __device__ __forceinline__ void static_block(const Ln& ln, T pp, uint32_t off, int hsum,
                                              St& st) {
-    block_batches<J, R, B, 0>(ln, pp, off, hsum, st);
+    if constexpr (St::kStaged) {
+        constexpr int kHmax = stage_hmax(J, R, int(sizeof(T)));
+        if (hsum <= kHmax) {
+            st.block_start(st.q);
+            block_batches<J, R, B, true, 0>(ln, pp, off, hsum, st);
+        } else {
+            block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st);
+        }
+    } else {
+        block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st);
+    }
     st.q += Simplex<J, R>::C;
 }
 
@@ -393,16 +413,18 @@ void k_eval(Params<T> prm) {
         const cptr<GridDesc::Stages> tb = &G->stage[sizeof(T) == 8 ? 0 : 1];
         st.lds = (lds_byte*)stage_buf;
         st.leaf_lds = as_const(tb->leaf_lds);
-        st.stage_end = as_const(tb->stage_end);
+        st.stage_first = as_const(tb->stage_first);
         st.seg_begin = as_const(tb->seg_begin);
         st.segs = as_const(tb->segs);
         st.num_stages = tb->num_stages;
         st.stage = 0;
-        st.cur = 0;
-        st.issue(0, 0);
-        __syncthreads();
-        st.qend = st.stage_end[0];
-        if (st.num_stages > 1) st.issue(1, uint32_t(kStageBytes));
+        st.next_first = -1;
+        if (st.num_stages > 0) {
+            st.issue(0);
+            __syncthreads();
+            st.next_first = st.stage_first[1];
+            if (st.num_stages > 1) st.issue(1);
+        }
     }
     Walker<D - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
     if (live) prm.out[G->out_off + pc] = st.acc.value();

@@ -1,6 +1,8 @@
 // Host-side grid construction, bijection and launch tables; see sg_grid.hpp.
 #include "sg_grid.hpp"
 
+#include <array>
+
 #include <algorithm>
 #include <cstring>
 
@@ -283,8 +285,9 @@ struct DfsWalk {
     int J;
     std::vector<uint8_t>* codes;
     std::vector<int32_t>* leaves;
-    std::vector<int32_t>* batches = nullptr;  // gather batch sizes, walk order
-    int B = 1;
+    // static blocks in walk order: first leaf, budget R, outer level sum
+    std::vector<std::array<int32_t, 3>>* blocks = nullptr;
+    int64_t nleaf = 0;
     int lv[kMaxDims] = {};
     bool ok = true;
 
@@ -337,6 +340,7 @@ struct DfsWalk {
         int64_t id = subspace_id(g, lv);
         if (id < 0) { ok = false; return; }
         if (leaves) leaves->push_back(g.sub_base[id]);
+        ++nleaf;
     }
 
     // budget R of the static block below a depth-J node, or -1: for an explicit set
@@ -352,15 +356,14 @@ struct DfsWalk {
 
     void go(int k, int rem) {
         int b = bound_at(k, rem);
-        if (J > 0 && k == J - 1 && (codes || batches)) {
+        if (J > 0 && k == J - 1 && (codes || blocks)) {
             int R = block_budget(k, rem);
             if (R >= 0) {
                 if (codes) codes->push_back(static_cast<uint8_t>(R));
-                if (batches) {
-                    int64_t C = 1;
-                    for (int i = 1; i <= J; ++i) C = C * (R + i) / i;   // C(R + J, J)
-                    for (int64_t c = 0; c < C; c += B)
-                        batches->push_back(int32_t(std::min<int64_t>(B, C - c)));
+                if (blocks) {
+                    int hsum = 0;
+                    for (int j = J; j < g.d; ++j) hsum += lv[j];
+                    blocks->push_back({int32_t(nleaf), R, hsum});
                 }
                 block(k, R);
                 return;
@@ -371,12 +374,8 @@ struct DfsWalk {
         }
         for (int l = 0; l <= b; ++l) {
             lv[k] = l;
-            if (k == 0) {
-                leaf();
-                if (batches) batches->push_back(1);
-            } else {
-                go(k - 1, rem - l);
-            }
+            if (k == 0) leaf();
+            else go(k - 1, rem - l);
         }
         lv[k] = 0;
     }
@@ -403,10 +402,11 @@ int explicit_tab_dims(const Grid& g) {
 
 namespace {
 
-// Pack the walk's leaves into LDS stages for one element size (see StagePlan).
-void build_stage_plan(const Grid& g, const std::vector<int32_t>& batches, int elem,
-                      StagePlan& p) {
+// Pack the whole static blocks into LDS stages for one element size (see StagePlan).
+void build_stage_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 3>>& blocks,
+                      int elem, StagePlan& p) {
     p = StagePlan();
+    if (g.N * elem > int64_t(0x7fffffff)) return;
     // block size of each subspace, by base offset
     std::unordered_map<int32_t, int64_t> size_of;
     size_of.reserve(size_t(g.S) * 2);
@@ -414,38 +414,41 @@ void build_stage_plan(const Grid& g, const std::vector<int32_t>& batches, int el
         const int64_t end = id + 1 < g.S ? g.sub_base[id + 1] : g.N;
         size_of[g.sub_base[id]] = end - g.sub_base[id];
     }
-    p.leaf_lds.assign(size_t(g.S), 0);
-    p.seg_begin.push_back(0);
-    int64_t q = 0;
-    int32_t used = 0;
-    for (int32_t nb : batches) {
-        int32_t bytes = 0;
-        for (int32_t c = 0; c < nb; ++c) {
-            const int64_t b = size_of[g.dfs_base[q + c]] * elem;
-            if (b > kStageMaxBlock) return;                     // not stageable
-            bytes += int32_t((b + 15) & ~int64_t(15));
-        }
-        if (bytes > kStageBytes) return;
-        if (used + bytes > kStageBytes) {                       // close the stage
-            p.stage_end.push_back(int32_t(q));
+    p.leaf_lds.assign(size_t(g.S), -1);
+    int32_t used = kStageBytes;          // no stage open yet
+    int buf = -1;
+    for (const auto& blk : blocks) {
+        const int32_t q0 = blk[0], R = blk[1], hsum = blk[2];
+        if (!stage_whole(J, R, hsum, elem)) continue;
+        const int64_t C = stage_binom(R + J, J);
+        int64_t bytes = 0;
+        for (int64_t c = 0; c < C; ++c)
+            bytes += (size_of[g.dfs_base[q0 + c]] * elem + 15) & ~int64_t(15);
+        if (bytes > kStageBytes) return;                 // the bound above is violated
+        if (used + bytes > kStageBytes) {                // open the next stage
+            buf ^= 1;
+            if (buf < 0) buf = 0;
+            p.stage_first.push_back(q0);
             p.seg_begin.push_back(int32_t(p.segs.size() / 2));
             used = 0;
         }
-        for (int32_t c = 0; c < nb; ++c, ++q) {
-            const int32_t base = g.dfs_base[q];
+        for (int64_t c = 0; c < C; ++c) {
+            const int32_t base = g.dfs_base[q0 + c];
             const int32_t b = int32_t(size_of[base] * elem);
-            p.leaf_lds[q] = used;
+            const int32_t at = buf * kStageBytes + used;
+            p.leaf_lds[q0 + c] = at;
             for (int32_t o = 0; o < b; o += kStageSegBytes) {
                 const int32_t piece = std::min(kStageSegBytes, b - o);
                 p.segs.push_back(int32_t(int64_t(base) * elem + o));
-                p.segs.push_back((used + o) | (((piece + 15) / 16) << 16));
+                p.segs.push_back((at + o) | (((piece + 15) / 16) << 16));
             }
             used += (b + 15) & ~15;
         }
     }
-    p.stage_end.push_back(int32_t(q));
+    p.stage_first.push_back(int32_t(g.S));
     p.seg_begin.push_back(int32_t(p.segs.size() / 2));
-    p.ok = q == g.S && g.N * elem <= int64_t(0x7fffffff);
+    p.segs.insert(p.segs.end(), 8, 0);                   // 4 empty segments of padding
+    p.ok = true;
 }
 
 }  // namespace
@@ -463,8 +466,8 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
         set_error("build_launch_tables: depth-first walk does not cover the level set");
         return kInvalid;
     }
-    // staged walk: batches of the walk with kStageJ tabled dimensions (levels of the
-    // tabled dimensions below 8), packed into LDS stages
+    // staged walk: the static blocks of the walk with kStageJ tabled dimensions (levels
+    // of the tabled dimensions below 8), packed into LDS stages
     g.stage[0] = StagePlan();
     g.stage[1] = StagePlan();
     {
@@ -472,16 +475,13 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
         bool tabled = J > 0;
         for (int j = 0; j < J; ++j) tabled = tabled && g.max_level[j] <= 7 && g.limits[j] <= 7;
         if (tabled) {
-            std::vector<int32_t> batches;
+            std::vector<std::array<int32_t, 3>> blocks;
             DfsWalk bw{g, J, nullptr, nullptr};
-            bw.batches = &batches;
-            bw.B = kStageBatch;
+            bw.blocks = &blocks;
             bw.go(d - 1, g.n - 1);
-            int64_t total = 0;
-            for (int32_t b : batches) total += b;
-            if (bw.ok && total == g.S) {
-                build_stage_plan(g, batches, 8, g.stage[0]);
-                build_stage_plan(g, batches, 4, g.stage[1]);
+            if (bw.ok && bw.nleaf == g.S) {
+                build_stage_plan(g, J, blocks, 8, g.stage[0]);
+                build_stage_plan(g, J, blocks, 4, g.stage[1]);
             }
         }
     }

@@ -29,16 +29,47 @@ constexpr int kMaxLevel = 15;       // per-dimension level cap of the kernels' t
 constexpr int kMaxLevelSum = 26;    // keeps 2^s * sizeof(double) inside 32-bit offsets
 constexpr int kExplicitTabDims = 3; // register-tabled dims of the explicit-set walk
 
-// Staged evaluation (surplus blocks copied into LDS by the workgroup, then read by each
-// lane): two stage buffers of kStageBytes per workgroup, blocks of at most
-// kStageMaxBlock bytes, copied in segments of at most kStageSegBytes (one wave
-// instruction of 16 bytes per lane).  The staged walk tables kStageJ dimensions and
-// issues its reads in batches of kStageBatch leaves.
-constexpr int kStageBytes = 16384;
-constexpr int kStageMaxBlock = 4096;
+#if defined(__HIPCC__)
+#define SG_HOST_DEVICE __host__ __device__
+#else
+#define SG_HOST_DEVICE
+#endif
+
+// Staged evaluation: the workgroup copies surplus blocks into LDS and each lane reads its
+// coefficient there.  Two stage buffers of kStageBytes per workgroup; blocks are copied
+// in segments of at most kStageSegBytes (one wave instruction of 16 bytes per lane).
+// The staged walk tables kStageJ dimensions and reads in batches of kStageBatch leaves.
+// What is staged: the static blocks of the walk (all leaves below a depth-J node whose
+// remaining level budget R fits the tabled dimensions) small enough to share a stage
+// with others, stage_whole(); other leaves are gathered from global memory.
+constexpr int kStageBytes = 24576;
 constexpr int kStageSegBytes = 1024;
 constexpr int kStageJ = 3;
 constexpr int kStageBatch = 8;
+
+SG_HOST_DEVICE constexpr int64_t stage_binom(int a, int b) {
+    int64_t r = 1;
+    for (int i = 1; i <= b; ++i) r = r * (a - b + i) / i;
+    return r;
+}
+// Bound on the LDS bytes of the static block of budget R over J tabled dimensions below
+// a node whose outer levels sum to hsum: sum over its C(R+J, J) leaves of
+// 2^(hsum + |c|) elements, plus 16 bytes of alignment padding per leaf.
+SG_HOST_DEVICE constexpr int64_t stage_block_bytes(int J, int R, int hsum, int elem) {
+    int64_t units = 0;                 // sum_c 2^|c| = sum_s C(s+J-1, J-1) 2^s
+    for (int s = 0; s <= R; ++s) units += stage_binom(s + J - 1, J - 1) << s;
+    return ((int64_t(elem) << hsum) * units) + 16 * stage_binom(R + J, J);
+}
+SG_HOST_DEVICE constexpr bool stage_whole(int J, int R, int hsum, int elem) {
+    return hsum <= 24 && stage_block_bytes(J, R, hsum, elem) <= kStageBytes;
+}
+// stage_whole grows false with hsum: the largest hsum for which it holds (-1: none)
+SG_HOST_DEVICE constexpr int stage_hmax(int J, int R, int elem) {
+    int h = -1;
+    for (int x = 0; x <= 24; ++x)
+        if (stage_whole(J, R, x, elem)) h = x;
+    return h;
+}
 
 struct HierTables {
     // One entry per tile / family / block of the pole-family decomposition of one
@@ -52,15 +83,16 @@ struct HierTables {
 };
 
 // How the staged walk copies one grid's surplus blocks into LDS, for one element size.
-// Leaves (subspaces in walk order) are packed into stages of at most kStageBytes, each
-// leaf at a 16-byte aligned offset; a stage ends only where a gather batch of the walk
-// ends, so the kernel checks for a stage change once per batch.
+// The whole static blocks (stage_whole) are packed in walk order into stages; stage k
+// lives in LDS buffer k % 2 and holds whole blocks only, so the kernel checks for a stage
+// change once per block.  Offsets are absolute LDS byte offsets (buffer included).
 struct StagePlan {
-    bool ok = false;                  // every block fits: the staged walk may run
-    std::vector<int32_t> leaf_lds;    // [S] byte offset of each leaf in its stage
-    std::vector<int32_t> stage_end;   // [K] first leaf after stage k
+    bool ok = false;                  // the staged walk may run on this grid
+    std::vector<int32_t> leaf_lds;    // [S] LDS byte offset of each staged leaf
+    std::vector<int32_t> stage_first; // [K+1] first leaf of stage k; [K] = S
     std::vector<int32_t> seg_begin;   // [K+1] first copy segment of stage k
-    std::vector<int32_t> segs;        // [nseg][2]: source byte offset, lds offset | lanes << 16
+    std::vector<int32_t> segs;        // [nseg + 4][2]: source byte offset,
+                                      //   lds offset | lanes << 16 (4 zero entries at the end)
 };
 
 struct Grid {

@@ -197,6 +197,64 @@ __global__ __launch_bounds__(kThreads) void k_grid_coords(BijParams p, double* c
     }
 }
 
+// asg.c:87-94 and its coordinates asg.c:148-151, in T, left to right, no contraction:
+// with T = float the value is bit-identical to the reference's init().
+template <class T>
+__device__ __forceinline__ T fct_reference(int d, const int32_t* lv, const int32_t* iv) {
+#pragma clang fp contract(off)
+    T res = T(0);
+    for (int k = 0; k < d; ++k) {
+        const T x = T(iv[k]) * T(1) / T(1u << lv[k]) + T(1) / T(1u << (lv[k] + 1));
+        res += x * x * (T(1) - x) * (T(1) - x);
+    }
+    return res;
+}
+
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_sample(BijParams p, SampleParams sp, T* out) {
+    __shared__ int32_t sdp[kMaxDims * kMaxN];
+    __shared__ int32_t slidx[kMaxN + 1];
+    __shared__ int32_t scnt[kMaxN + 1];
+    load_bij_tables(p, sdp, slidx, scnt);
+    int32_t lv[kMaxDims], iv[kMaxDims];
+    for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < p.N;
+         o += int64_t(gridDim.x) * blockDim.x) {
+        decode_offset(p, sdp, slidx, scnt, o, lv, iv);
+        T v;
+        if (sp.fn == kFnReference) {
+            v = fct_reference<T>(p.d, lv, iv);
+        } else {
+            // coordinates (2 i + 1) / 2^(l+1) are exact in T
+            double acc = sp.fn == kFnGaussianSum ? 0.0 : 1.0;
+            if (sp.fn == kFnGauss) {
+                double e = 0.0;
+                for (int k = 0; k < p.d; ++k) {
+                    const double x = ldexp(2.0 * iv[k] + 1.0, -(lv[k] + 1)) - sp.centre[0][k];
+                    e += x * x;
+                }
+                acc = exp(-e);
+            } else if (sp.fn == kFnGaussianSum) {
+                for (int g = 0; g < sp.terms; ++g) {
+                    double e = 0.0;
+                    for (int k = 0; k < p.d; ++k) {
+                        const double x = ldexp(2.0 * iv[k] + 1.0, -(lv[k] + 1)) - sp.centre[g][k];
+                        e += x * x / (2.0 * sp.width[g][k] * sp.width[g][k]);
+                    }
+                    acc += exp(-e);
+                }
+            } else {
+                for (int k = 0; k < p.d; ++k) {
+                    const double x = ldexp(2.0 * iv[k] + 1.0, -(lv[k] + 1));
+                    acc *= sp.fn == kFnParabola ? 4.0 * x * (1.0 - x) : sin(M_PI * x);
+                }
+                if (sp.fn == kFnSinProduct) acc *= sp.scale;
+            }
+            v = T(acc);
+        }
+        out[o] = v;
+    }
+}
+
 BijParams make_bij(const Grid& g, const DeviceTables& t) {
     BijParams p{};
     p.d = g.d;
@@ -475,6 +533,44 @@ __global__ __launch_bounds__(kThreads) void k_uniform(int64_t count, uint64_t se
     }
 }
 
+// The achievable-gather pattern (see launch_gather_probe_f64): lane p draws an xorshift32
+// stream and gathers element (r & (2^s - 1)) of each subspace block, 8 at a time, with the
+// same raw buffer loads (block base in the scalar offset) as the evaluation walk.
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_gather_probe(const T* alpha, int64_t N,
+                                                           const int32_t* base,
+                                                           const uint8_t* lsum, int S,
+                                                           int64_t P, uint32_t seed, T* out) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint32_t r = uint32_t(mix64(uint64_t(seed) << 32 | uint64_t(p))) | 1u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T*>(alpha), short(0), int(N * int64_t(sizeof(T))), 0x00020000);
+    const auto* cb = (const __attribute__((address_space(4))) int32_t*)base;
+    const auto* cl = (const __attribute__((address_space(4))) uint8_t*)lsum;
+    T acc = T(0);
+    for (int i = 0; i < S; i += 8) {
+        T v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            v[j] = T(0);
+            if (i + j < S) {
+                r ^= r << 13;
+                r ^= r >> 17;
+                r ^= r << 5;
+                const uint32_t off = (r & ((1u << cl[i + j]) - 1u)) * uint32_t(sizeof(T));
+                const uint32_t so = uint32_t(cb[i + j]) * uint32_t(sizeof(T));
+                if constexpr (sizeof(T) == 8)
+                    v[j] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, off, so, 0));
+                else
+                    v[j] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, off, so, 0));
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc += v[j];
+    }
+    if (p < P) out[p] = acc;
+}
+
 __global__ __launch_bounds__(kThreads) void k_f64_to_f32(const double* in, float* out,
                                                          int64_t count) {
     for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < count;
@@ -536,6 +632,12 @@ int upload_tables(const Grid& g, DeviceTables& t) {
     if ((st = upload(t.cnt, cnt32.data(), cnt32.size())) != kOk) return st;
     if ((st = upload(t.sub_levels, g.sub_levels.data(), g.sub_levels.size())) != kOk) return st;
     if ((st = upload(t.sub_base, g.sub_base.data(), g.sub_base.size())) != kOk) return st;
+    {
+        std::vector<uint8_t> ls(size_t(g.S), 0);
+        for (int64_t i = 0; i < g.S; ++i)
+            for (int j = 0; j < g.d; ++j) ls[size_t(i)] = uint8_t(ls[size_t(i)] + g.sub_levels[size_t(i * g.d + j)]);
+        if ((st = upload(t.sub_lsum, ls.data(), ls.size())) != kOk) return st;
+    }
     if ((st = upload(t.dfs_base, g.dfs_base.data(), g.dfs_base.size())) != kOk) return st;
     if ((st = upload(t.dfs_bound, g.dfs_bound.data(), g.dfs_bound.size())) != kOk) return st;
     eval::GridDesc desc{};
@@ -547,20 +649,20 @@ int upload_tables(const Grid& g, DeviceTables& t) {
     for (int e = 0; e < 2; ++e) {
         const StagePlan& p = g.stage[e];
         if (!p.ok) continue;
-        // one allocation: leaf_lds | stage_end | seg_begin | segs
+        // one allocation: leaf_lds | stage_first | seg_begin | segs
         std::vector<int32_t> all;
-        all.reserve(p.leaf_lds.size() + p.stage_end.size() + p.seg_begin.size() + p.segs.size());
+        all.reserve(p.leaf_lds.size() + p.stage_first.size() + p.seg_begin.size() + p.segs.size());
         all.insert(all.end(), p.leaf_lds.begin(), p.leaf_lds.end());
-        all.insert(all.end(), p.stage_end.begin(), p.stage_end.end());
+        all.insert(all.end(), p.stage_first.begin(), p.stage_first.end());
         all.insert(all.end(), p.seg_begin.begin(), p.seg_begin.end());
         all.insert(all.end(), p.segs.begin(), p.segs.end());
         if ((st = upload(t.stage_tabs[e], all.data(), all.size())) != kOk) return st;
         eval::GridDesc::Stages& sd = desc.stage[e];
         sd.leaf_lds = t.stage_tabs[e];
-        sd.stage_end = sd.leaf_lds + p.leaf_lds.size();
-        sd.seg_begin = sd.stage_end + p.stage_end.size();
+        sd.stage_first = sd.leaf_lds + p.leaf_lds.size();
+        sd.seg_begin = sd.stage_first + p.stage_first.size();
         sd.segs = sd.seg_begin + p.seg_begin.size();
-        sd.num_stages = int32_t(p.stage_end.size());
+        sd.num_stages = int32_t(p.stage_first.size()) - 1;
     }
     t.host_desc = desc;
     eval::GridDesc* ddesc = nullptr;
@@ -590,6 +692,7 @@ void free_tables(DeviceTables& t) {
     free_dev(t.cnt);
     free_dev(t.sub_levels);
     free_dev(t.sub_base);
+    free_dev(t.sub_lsum);
     free_dev(t.dfs_base);
     free_dev(t.dfs_bound);
     free_dev(t.stage_tabs[0]);
@@ -676,6 +779,49 @@ int launch_uniform_points_f32(int64_t count, uint64_t seed, float* out, hipStrea
                        seed, out);
     return check_launch("k_uniform");
 }
+template <class T>
+static int gather_probe(const Grid& g, const DeviceTables& t, const T* alpha, int64_t P,
+                        uint32_t seed, T* out, hipStream_t s) {
+    if (P <= 0) return kOk;
+    if (g.N * int64_t(sizeof(T)) > int64_t(0x7fffffff) || (P + kThreads - 1) / kThreads > 0x7fffffff) {
+        set_error("gather probe: grid or point count out of range");
+        return kRange;
+    }
+    hipLaunchKernelGGL(k_gather_probe<T>, dim3(unsigned((P + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, s, alpha, g.N, t.sub_base, t.sub_lsum, int(g.S), P,
+                       seed, out);
+    return check_launch("k_gather_probe");
+}
+int launch_gather_probe_f64(const Grid& g, const DeviceTables& t, const double* alpha,
+                            int64_t P, uint32_t seed, double* out, hipStream_t s) {
+    return gather_probe(g, t, alpha, P, seed, out, s);
+}
+int launch_gather_probe_f32(const Grid& g, const DeviceTables& t, const float* alpha,
+                            int64_t P, uint32_t seed, float* out, hipStream_t s) {
+    return gather_probe(g, t, alpha, P, seed, out, s);
+}
+
+template <class T>
+static int sample(const Grid& g, const DeviceTables& t, const SampleParams& sp, T* out,
+                  hipStream_t s) {
+    if (sp.fn < kFnReference || sp.fn > kFnSinProduct || sp.terms < 1 ||
+        sp.terms > kMaxSampleTerms) {
+        set_error("sample: unknown function or term count");
+        return kInvalid;
+    }
+    hipLaunchKernelGGL(k_sample<T>, dim3(grid_for(g.N)), dim3(kThreads), 0, s, make_bij(g, t),
+                       sp, out);
+    return check_launch("k_sample");
+}
+int launch_sample_f64(const Grid& g, const DeviceTables& t, const SampleParams& sp, double* out,
+                      hipStream_t s) {
+    return sample(g, t, sp, out, s);
+}
+int launch_sample_f32(const Grid& g, const DeviceTables& t, const SampleParams& sp, float* out,
+                      hipStream_t s) {
+    return sample(g, t, sp, out, s);
+}
+
 int launch_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s) {
     if (count == 0) return kOk;
     hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(count)), dim3(kThreads), 0, s, in, out, count);

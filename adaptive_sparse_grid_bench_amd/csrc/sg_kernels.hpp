@@ -26,10 +26,10 @@ struct GridDesc {
     int32_t n;                  // level sums 0..n-1
     int32_t limits[kMaxDims];
     // LDS stage tables of the staged walk ([0]: 8-byte surpluses, [1]: 4-byte), see
-    // StagePlan; num_stages = 0 when the grid cannot be staged
+    // StagePlan
     struct Stages {
         const int32_t* leaf_lds;
-        const int32_t* stage_end;
+        const int32_t* stage_first;
         const int32_t* seg_begin;
         const int32_t* segs;
         int32_t num_stages;
@@ -55,6 +55,7 @@ struct DeviceTables {
     int32_t* cnt = nullptr;         // [n+1]
     uint8_t* sub_levels = nullptr;  // [S][d]
     int32_t* sub_base = nullptr;    // [S]
+    uint8_t* sub_lsum = nullptr;    // [S] level sum of each subspace (compact order)
     int32_t* dfs_base = nullptr;    // [S]
     uint8_t* dfs_bound = nullptr;   // explicit sets: walk-node bounds
     int32_t* stage_tabs[2] = {};    // StagePlan arrays per element size, one allocation each
@@ -94,11 +95,15 @@ struct EvalTuning {
     int bin_min_points = 4096;  // spatially bin the points of launches at least this big
 };
 
-// default variants by dimension and precision (measured at d = 10, 12, 16, 20)
-inline int default_variant(int d, bool f32) { return f32 ? 2 : (d >= 16 ? 1 : 0); }
-
 // the walk variant that reads surpluses from LDS stages
 constexpr int kStagedVariant = 3;
+
+// default variants by dimension and precision, from timings at 2^22-2^24 points on
+// d = 5, 10, 12, 16, 20 (the autotuner refines them per grid shape)
+inline int default_variant(int d, bool f32) {
+    if (f32) return d <= 6 ? kStagedVariant : 2;
+    return d >= 16 ? 1 : kStagedVariant;
+}
 // the variant a launch runs: the staged walk falls back to variant 0 for a grid whose
 // blocks cannot be staged in this element size
 inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
@@ -132,5 +137,34 @@ int launch_evaluate_ref_f32(const Grid& g, const DeviceTables& t, const float* a
 int launch_uniform_points_f64(int64_t count, uint64_t seed, double* out, hipStream_t s);
 int launch_uniform_points_f32(int64_t count, uint64_t seed, float* out, hipStream_t s);
 int launch_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s);
+
+// Test functions sampled at the grid points on the device (values in compact order).
+enum SampleFn : int {
+    kFnReference = 0,     // sum_k x_k^2 (1 - x_k)^2, asg.c:87-94, same operation order
+    kFnParabola = 1,      // prod_k 4 x_k (1 - x_k)
+    kFnGauss = 2,         // exp(-sum_k (x_k - c_k)^2)
+    kFnGaussianSum = 3,   // sum_g exp(-sum_k (x_k - c_gk)^2 / (2 w_gk^2))
+    kFnSinProduct = 4,    // scale * prod_k sin(pi x_k)
+};
+constexpr int kMaxSampleTerms = 8;
+struct SampleParams {
+    int fn = 0;
+    int terms = 1;                    // Gaussians of kFnGaussianSum
+    double scale = 1.0;               // kFnSinProduct factor
+    double centre[kMaxSampleTerms][kMaxDims] = {};
+    double width[kMaxSampleTerms][kMaxDims] = {};
+};
+int launch_sample_f64(const Grid& g, const DeviceTables& t, const SampleParams& sp, double* out,
+                      hipStream_t s);
+int launch_sample_f32(const Grid& g, const DeviceTables& t, const SampleParams& sp, float* out,
+                      hipStream_t s);
+
+// The achievable-gather pattern of grid g: per point, one gather at a random position
+// inside every subspace block, blocks in compact (level-sum) order, in batches of 8;
+// no hat products and no walk.  out[p] = the sum of the values gathered for point p.
+int launch_gather_probe_f64(const Grid& g, const DeviceTables& t, const double* alpha,
+                            int64_t P, uint32_t seed, double* out, hipStream_t s);
+int launch_gather_probe_f32(const Grid& g, const DeviceTables& t, const float* alpha,
+                            int64_t P, uint32_t seed, float* out, hipStream_t s);
 
 }  // namespace sg

@@ -26,6 +26,10 @@ SG_E_HIP = -3
 SG_E_RANGE = -4
 SG_E_UNSUPPORTED = -5
 
+# test functions of sg_sample_* (include/sgrid.h)
+SAMPLE_FUNCTIONS = {"reference": 0, "parabola": 1, "gauss": 2, "gaussian_sum": 3,
+                    "sin_product": 4}
+
 # every symbol include/sgrid.h declares
 EXPORTS = [
     "sg_last_error", "sg_status_string", "sg_version",
@@ -36,6 +40,7 @@ EXPORTS = [
     "sg_hierarchize_f64", "sg_hierarchize_f32",
     "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
     "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
+    "sg_gather_probe_f64", "sg_gather_probe_f32", "sg_sample_f64", "sg_sample_f32",
     "sg_batch_create", "sg_batch_destroy", "sg_batch_values", "sg_batch_set_points",
     "sg_batch_hierarchize_f64", "sg_batch_hierarchize_f32", "sg_batch_evaluate_f64",
     "sg_batch_evaluate_f32c", "sg_batch_run_graph_f64", "sg_batch_run_graph_f32c",
@@ -113,6 +118,10 @@ def _load() -> C.CDLL:
         "sg_uniform_points_f64": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_uniform_points_f32": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_convert_f64_to_f32": (i, [vp, vp, ll, vp]),
+        "sg_sample_f64": (i, [vp, i, C.POINTER(C.c_double), ll, vp, vp]),
+        "sg_sample_f32": (i, [vp, i, C.POINTER(C.c_double), ll, vp, vp]),
+        "sg_gather_probe_f64": (i, [vp, vp, ll, C.c_ulonglong, vp, vp]),
+        "sg_gather_probe_f32": (i, [vp, vp, ll, C.c_ulonglong, vp, vp]),
         "sg_batch_create": (i, [C.POINTER(vp), i, C.POINTER(vp)]),
         "sg_batch_destroy": (i, [vp]),
         "sg_batch_values": (i, [vp, llp, llp]),
@@ -281,6 +290,16 @@ class Grid:
         fn = lib.sg_hierarchize_f64 if values.dtype == torch.float64 else lib.sg_hierarchize_f32
         check(fn(self._h, _ptr(values), _stream(stream)))
 
+    def sample(self, function: str, out, params=None, stream=None) -> None:
+        """out[o] = function at grid point o, on the device (asg.c:143-152).  function is
+        a SAMPLE_FUNCTIONS name; params as include/sgrid.h states (a flat float sequence)."""
+        import torch
+        fn = lib.sg_sample_f64 if out.dtype == torch.float64 else lib.sg_sample_f32
+        p = np.ascontiguousarray(params if params is not None else [], np.float64)
+        check(fn(self._h, SAMPLE_FUNCTIONS[function],
+                 p.ctypes.data_as(C.POINTER(C.c_double)) if p.size else None, p.size,
+                 _ptr(out), _stream(stream)))
+
     def evaluate(self, alpha, coords, out, stream=None, mode: str = "fast") -> None:
         """out[p] = interpolant at coords[p] (asg.c:1237-1274).
 
@@ -297,6 +316,15 @@ class Grid:
         else:
             raise ValueError(mode)
         check(fn(self._h, _ptr(alpha), _ptr(coords), P, _ptr(out), _stream(stream)))
+
+
+def gather_probe(grid: "Grid", alpha, out, seed: int = 1, stream=None) -> None:
+    """The achievable-gather pattern of grid's shape for out.numel() points (one random
+    in-block gather per subspace per point, no hats): the evaluate kernels' gather
+    roofline.  out[p] = sum of point p's gathered values."""
+    import torch
+    fn = lib.sg_gather_probe_f64 if alpha.dtype == torch.float64 else lib.sg_gather_probe_f32
+    check(fn(grid._h, _ptr(alpha), out.numel(), seed, _ptr(out), _stream(stream)))
 
 
 def uniform_points(out, seed: int = 42, stream=None) -> None:

@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "eval pts/s & hierarchize GP/s at d=10,n=8; served p50/p99 vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+VARIANT_NAMES = {0: "gather, 3 tabled dims, batches of 8", 1: "gather, 2 tabled dims",
+                 2: "gather, batches of 4, 4 waves/SIMD",
+                 3: "staged in LDS, 3 tabled dims, batches of 8"}
 
 
 def cpu_baseline(d: int, n: int, P_sample: int, seed: int) -> dict:
@@ -73,6 +76,54 @@ def cpu_baseline(d: int, n: int, P_sample: int, seed: int) -> dict:
                           f"at d={d} n={n} ({exc})", "host_cpu": _cpu_model()}
 
 
+def precision_form(args) -> dict:
+    """d=20, n=6 (N=1,515,009): a seeded sum of 8 anisotropic Gaussians (centres U[0.2,
+    0.8]^20, widths log-uniform [0.05, 0.5]) hierarchized in FP64 on the GPU, evaluated at
+    2^24 uniform points in FP32 with Kahan accumulation and in FP64; max and RMS error of
+    the FP32 result against the FP64 walk (which the tests hold to the CPU FP64 oracle
+    within 1e-12 relative to sum |terms|)."""
+    import torch
+    from adaptive_sparse_grid_bench_amd import sgrid
+    from adaptive_sparse_grid_bench_amd import workloads as W
+    d, n, P = 20, 6, 1 << args.log2_points
+    g = sgrid.Grid(d, n)
+    x = torch.empty((g.N, d), dtype=torch.float64, device="cuda")
+    g.grid_coords(x)
+    c, w = W.gaussian_sum_params(d, 8, seed=7)
+    f = torch.zeros(g.N, dtype=torch.float64, device="cuda")
+    for cc, ww in zip(c, w):
+        cc = torch.from_numpy(cc).cuda()
+        ww = torch.from_numpy(ww).cuda()
+        f += torch.exp(-(((x - cc) ** 2) / (2 * ww * ww)).sum(1))
+    del x
+    g.hierarchize(f)
+    pts = torch.empty((P, d), dtype=torch.float64, device="cuda")
+    sgrid.uniform_points(pts, seed=11)
+    o64 = torch.empty(P, dtype=torch.float64, device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    g.evaluate(f, pts, o64)
+    ev[1].record()
+    a32, p32 = f.float(), pts.float()
+    del pts
+    o32 = torch.empty(P, dtype=torch.float32, device="cuda")
+    ev[2].record()
+    g.evaluate(a32, p32, o32)
+    ev[3].record()
+    torch.cuda.synchronize()
+    err = (o32.double() - o64).abs()
+    umax = o64.abs().max().item()
+    res = {"workload": f"d=20 n=6 (N={g.N}, S={g.S}), sum of 8 Gaussians, 2^{args.log2_points} "
+                       "points; FP32 Kahan vs FP64 walk",
+           "f64_ms": ev[0].elapsed_time(ev[1]), "f32c_ms": ev[2].elapsed_time(ev[3]),
+           "max_abs_err": err.max().item(), "rms_err": err.pow(2).mean().sqrt().item(),
+           "max_abs_u": umax}
+    res["max_rel_to_max_u"] = res["max_abs_err"] / umax if umax > 0 else None
+    del o64, o32, a32, p32, f, err
+    torch.cuda.empty_cache()
+    return res
+
+
 def other_forms(g, alpha, pts, out64, P, args) -> dict:
     """FP32-Kahan evaluation of the same workload, the batched form (256 grids, one
     hipGraph launch vs per-grid launches) and the served form (continuous batching vs
@@ -95,6 +146,7 @@ def other_forms(g, alpha, pts, out64, P, args) -> dict:
                    "max_abs_err_vs_f64": err.max().item(),
                    "rms_err_vs_f64": err.pow(2).mean().sqrt().item()}
     del a32, p32, o32
+    res["precision"] = precision_form(args)
     parts = batched.build(256, tune=True)
     res["batched"] = batched.measure(*parts)
     del parts
@@ -202,13 +254,34 @@ def main() -> int:
     # the point's coordinates and its output value
     bytes_per_point = g.S * 8 + d * 8 + 8
     achieved = P * bytes_per_point / (eval_avg * 1e-3) / 1e9
+    # HBM bytes per launch of this kernel, measured with rocprofv3 counters
+    # (FETCH_SIZE scaled by a plain copy's reading, plus WRITE_SIZE)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "eval_dfs_d10n8_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"eval_d{d}n{n}_traffic.json")
     if os.path.exists(tpath):
         try:
-            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+            tj = json.load(open(tpath))
+            if tj.get("num_points") == P:
+                traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
+    variant = g.get_tuning()["eval_variant_f64"]
+
+    # achievable gather rate of this grid's shape: one random in-block 8-byte gather per
+    # subspace per point, blocks in level-sum order, no hats (the gather roofline)
+    probe_out = torch.empty(P, dtype=torch.float64, device=dev)
+    pev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    probe_ms = []
+    for _ in range(3):
+        pev[0].record(stream)
+        sgrid.gather_probe(g, alpha, probe_out, seed=7, stream=stream)
+        pev[1].record(stream)
+        torch.cuda.synchronize()
+        probe_ms.append(pev[0].elapsed_time(pev[1]))
+    del probe_out
+    probe_best = min(probe_ms[1:])
+    gather_bytes = P * g.S * 8
+    achievable = gather_bytes / (probe_best * 1e-3) / 1e9
 
     rec = {
         "metric": METRIC,
@@ -235,7 +308,7 @@ def main() -> int:
         "evaluate_ms": eval_avg,
         "evaluate_pts_per_s_kernel": P / (eval_avg * 1e-3),
         "roofline": {
-            "kernel": "k_eval_dfs<10,8,double>",
+            "kernel": f"k_eval<{d},...,double> walk variant {variant} ({VARIANT_NAMES[variant]})",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -243,6 +316,14 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "bytes_per_point": bytes_per_point,
+            "achievable_gather": {
+                "GBps": achievable,
+                "probe_ms": probe_best,
+                "evaluate_gather_GBps": gather_bytes / (eval_avg * 1e-3) / 1e9,
+                "frac": probe_best / eval_avg,
+                "pattern": "sg_gather_probe_f64: per point one random 8-byte gather inside "
+                           "every subspace block, level-sum order, 8 in flight per lane",
+            },
         },
         "cpu_baseline": None,
     }

@@ -44,7 +44,7 @@ typedef struct sg_grid sg_grid;
 
 const char *sg_last_error(void);
 const char *sg_status_string(int status);
-int sg_version(void);
+int sg_version(void);   /* changes whenever kernel behaviour or timing may change */
 
 /* ------------------------------------------------------------------ grids ---- */
 
@@ -88,7 +88,7 @@ int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *bas
  *                     lanes read them there.  A grid with blocks over 4 KiB (or tabled
  *                     dimensions above level 7) runs variant 0 instead of 3;
  *   bin_min_points    evaluate launches with at least this many points bin the points
- *                     spatially first (<= 0: never);
+ *                     spatially first (<= 0: never; the staged variant never bins);
  *   hier_tile_values  target values per hierarchization workgroup tile (>= 64). */
 #define SG_EVAL_VARIANTS 4
 typedef struct sg_tuning {
@@ -226,6 +226,29 @@ int sg_server_stats(sg_server *server, long long *launches, long long *points_do
 
 /* ----------------------------------------------------------- inputs --------- */
 
+/* Sample a test function at every grid point on the device: d_values[o] = f(x(o)) with
+ * x_k = (2 i_k + 1) / 2^(l_k + 1) and (l, i) = idx2gp(o), compact order.  Replaces the
+ * host sampling loop of init() (asg.c:143-152) and init_sg1d (asg.cu:105-117).
+ *   SG_FN_REFERENCE     sum_k x_k^2 (1 - x_k)^2, the reference's fct (asg.c:87-94) in the
+ *                       same operation order: sg_sample_f32 is bit-identical to init();
+ *                       params: none
+ *   SG_FN_PARABOLA      prod_k 4 x_k (1 - x_k); params: none
+ *   SG_FN_GAUSS         exp(-sum_k (x_k - c_k)^2); params: none (c = 1/2) or c[d]
+ *   SG_FN_GAUSSIAN_SUM  sum_g exp(-sum_k (x_k - c_gk)^2 / (2 w_gk^2)), g < G <= 8;
+ *                       params: [G, c[G][d], w[G][d]]
+ *   SG_FN_SIN_PRODUCT   a * prod_k sin(pi x_k); params: none (a = 1) or [a]
+ * All but SG_FN_REFERENCE are computed in FP64 and rounded to the output type; exp and
+ * sin are the device library's, within a few ulp of the host's. */
+#define SG_FN_REFERENCE 0
+#define SG_FN_PARABOLA 1
+#define SG_FN_GAUSS 2
+#define SG_FN_GAUSSIAN_SUM 3
+#define SG_FN_SIN_PRODUCT 4
+int sg_sample_f64(const sg_grid *grid, int function, const double *params, long long num_params,
+                  double *d_values, void *stream);
+int sg_sample_f32(const sg_grid *grid, int function, const double *params, long long num_params,
+                  float *d_values, void *stream);
+
 /* count values k * 2^-24, k = top 24 bits of splitmix64(seed * 2^32 + j), j = 0..count-1:
  * seeded uniform query coordinates exact in FP32 and FP64 (replaces the time-seeded
  * init_coord_mat, asg.cu:152-165). */
@@ -235,6 +258,19 @@ int sg_uniform_points_f32(long long count, unsigned long long seed, float *d_out
                           void *stream);
 /* FP64 -> FP32 rounding copy (surpluses for the FP32 evaluate). */
 int sg_convert_f64_to_f32(const double *d_in, float *d_out, long long count, void *stream);
+
+/* ------------------------------------------------------ measurement --------- */
+
+/* The achievable-gather pattern of this grid's shape, the gather roofline the evaluate
+ * kernels are measured against.  Per point, one 8-byte (_f64) or 4-byte (_f32) gather at
+ * a random position (xorshift32 stream seeded by seed and the point index) inside every
+ * subspace block, blocks in compact (level-sum) order, 8 gathers in flight per lane, no
+ * hat products and no walk.  d_out[p] = sum of point p's gathered values.  No
+ * counterpart in the reference (it has no roofline measurement). */
+int sg_gather_probe_f64(const sg_grid *grid, const double *d_alpha, long long num_points,
+                        unsigned long long seed, double *d_out, void *stream);
+int sg_gather_probe_f32(const sg_grid *grid, const float *d_alpha, long long num_points,
+                        unsigned long long seed, float *d_out, void *stream);
 
 #ifdef __cplusplus
 }

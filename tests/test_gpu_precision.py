@@ -53,7 +53,7 @@ def test_precision_vs_cpu_oracle_sample(grid_d20, torch_gpu):
     g.evaluate(a.float(), torch.from_numpy(pts.astype(np.float32)).cuda(), o32)
     err = np.abs(o32.cpu().numpy().astype(np.float64) - ref)
     print(f"\nprecision d=20 n=6 vs CPU FP64 oracle, {P} points: max {err.max():.3e} "
-          f"rms {np.sqrt(np.mean(err ** 2)):.3e} (|u| max {np.abs(ref).max():.3f})")
+          f"rms {np.sqrt(np.mean(err ** 2)):.3e} (|u| max {np.abs(ref).max():.3e})")
     assert err.max() <= BOUND
 
 

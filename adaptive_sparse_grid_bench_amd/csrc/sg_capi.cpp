@@ -2,10 +2,12 @@
 // dispatch to the host tables (sg_grid.cpp) and the HIP launchers (sg_kernels.hip).
 #include <hip/hip_runtime_api.h>
 
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/sgrid.h"
 #include "sg_batch.hpp"
@@ -415,6 +417,146 @@ int sg_sample_f32(const sg_grid* grid, int function, const double* params, long 
     return sg::launch_sample_f32(grid->g, grid->t, sp, d_values, as_stream(stream));
 }
 
+// ------------------------------------------------------------- grid files --
+// Layout (little-endian): 64-byte header, then the level set (restricted-regular:
+// int32 limits[d]; explicit: uint8 levels[S][d] in compact order), then N values.
+namespace {
+
+constexpr char kFileMagic[8] = {'S', 'G', 'R', 'I', 'D', 'v', '0', '1'};
+
+struct FileHeader {
+    char magic[8];
+    uint32_t header_bytes;
+    uint32_t d, n;
+    uint32_t kind;              // 0 restricted-regular, 1 explicit
+    uint32_t elem_bytes;        // 8, 4, or 0 (no values)
+    uint32_t reserved;
+    uint64_t N, S;
+    uint64_t checksum;          // FNV-1a 64 of every byte after the header
+    uint64_t reserved2;
+};
+static_assert(sizeof(FileHeader) == 64, "file header is 64 bytes");
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h;
+}
+
+struct FileCloser {
+    std::FILE* f;
+    ~FileCloser() { if (f) std::fclose(f); }
+};
+
+int read_header(std::FILE* f, const char* path, FileHeader& h) {
+    if (std::fread(&h, sizeof(h), 1, f) != 1)
+        return fail(sg::kInvalid, std::string("sg_file: short header in ") + path);
+    if (std::memcmp(h.magic, kFileMagic, 8) != 0 || h.header_bytes != sizeof(FileHeader))
+        return fail(sg::kInvalid, std::string("sg_file: not a grid file: ") + path);
+    if (h.d < 1 || h.d > sg::kMaxDims || h.n < 1 || h.kind > 1 ||
+        (h.elem_bytes != 0 && h.elem_bytes != 4 && h.elem_bytes != 8) || h.N == 0 ||
+        h.S == 0 || h.N > uint64_t(1) << 31 || h.S > h.N)
+        return fail(sg::kInvalid, std::string("sg_file: bad header in ") + path);
+    return sg::kOk;
+}
+
+}  // namespace
+
+int sg_file_write(const char* path, const sg_grid* grid, const void* values, int elem_bytes) {
+    if (!path || !grid || (elem_bytes != 0 && elem_bytes != 4 && elem_bytes != 8) ||
+        (elem_bytes != 0 && !values))
+        return fail(sg::kInvalid, "sg_file_write: bad argument");
+    const sg::Grid& g = grid->g;
+    FileHeader h{};
+    std::memcpy(h.magic, kFileMagic, 8);
+    h.header_bytes = sizeof(FileHeader);
+    h.d = uint32_t(g.d);
+    h.n = uint32_t(g.n);
+    h.kind = g.explicit_set ? 1u : 0u;
+    h.elem_bytes = uint32_t(elem_bytes);
+    h.N = uint64_t(g.N);
+    h.S = uint64_t(g.S);
+    std::vector<int32_t> lim(g.limits, g.limits + g.d);
+    const void* shape = g.explicit_set ? static_cast<const void*>(g.sub_levels.data())
+                                       : static_cast<const void*>(lim.data());
+    const size_t shape_bytes = g.explicit_set ? g.sub_levels.size() : lim.size() * 4;
+    const size_t value_bytes = size_t(g.N) * size_t(elem_bytes);
+    uint64_t c = fnv1a(0xcbf29ce484222325ull, shape, shape_bytes);
+    h.checksum = fnv1a(c, values, value_bytes);
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(sg::kInvalid, std::string("sg_file_write: cannot open ") + path);
+    FileCloser fc{f};
+    if (std::fwrite(&h, sizeof(h), 1, f) != 1 || std::fwrite(shape, 1, shape_bytes, f) != shape_bytes ||
+        (value_bytes && std::fwrite(values, 1, value_bytes, f) != value_bytes))
+        return fail(sg::kInvalid, std::string("sg_file_write: write failed: ") + path);
+    if (std::fclose(f) != 0) {
+        fc.f = nullptr;
+        return fail(sg::kInvalid, std::string("sg_file_write: write failed: ") + path);
+    }
+    fc.f = nullptr;
+    return sg::kOk;
+}
+
+int sg_file_info(const char* path, int* num_dims, int* num_levels, long long* num_grid_points,
+                 long long* num_subspaces, int* elem_bytes) {
+    if (!path) return fail(sg::kInvalid, "sg_file_info: NULL path");
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(sg::kInvalid, std::string("sg_file_info: cannot open ") + path);
+    FileCloser fc{f};
+    FileHeader h{};
+    int st = read_header(f, path, h);
+    if (st != sg::kOk) return st;
+    if (num_dims) *num_dims = int(h.d);
+    if (num_levels) *num_levels = int(h.n);
+    if (num_grid_points) *num_grid_points = (long long)h.N;
+    if (num_subspaces) *num_subspaces = (long long)h.S;
+    if (elem_bytes) *elem_bytes = int(h.elem_bytes);
+    return sg::kOk;
+}
+
+int sg_file_read(const char* path, sg_grid** out, void* values, long long values_bytes) {
+    if (!path || !out) return fail(sg::kInvalid, "sg_file_read: NULL argument");
+    *out = nullptr;
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(sg::kInvalid, std::string("sg_file_read: cannot open ") + path);
+    FileCloser fc{f};
+    FileHeader h{};
+    int st = read_header(f, path, h);
+    if (st != sg::kOk) return st;
+    const size_t shape_bytes = h.kind == 1 ? size_t(h.S) * h.d : size_t(h.d) * 4;
+    const size_t value_bytes = size_t(h.N) * h.elem_bytes;
+    if (values && values_bytes < (long long)value_bytes)
+        return fail(sg::kRange, "sg_file_read: value buffer smaller than the file's values");
+    std::vector<unsigned char> shape(shape_bytes);
+    std::vector<unsigned char> vals(values ? value_bytes : 0);
+    if (std::fread(shape.data(), 1, shape_bytes, f) != shape_bytes)
+        return fail(sg::kInvalid, std::string("sg_file_read: truncated level set in ") + path);
+    uint64_t c = fnv1a(0xcbf29ce484222325ull, shape.data(), shape_bytes);
+    if (values) {
+        if (std::fread(vals.data(), 1, value_bytes, f) != value_bytes)
+            return fail(sg::kInvalid, std::string("sg_file_read: truncated values in ") + path);
+        c = fnv1a(c, vals.data(), value_bytes);
+        if (c != h.checksum)
+            return fail(sg::kInvalid, std::string("sg_file_read: checksum mismatch in ") + path);
+    }
+    sg_grid* g = nullptr;
+    if (h.kind == 1) {
+        st = sg_grid_create_explicit(int(h.d), shape.data(), (long long)h.S, &g);
+    } else {
+        std::vector<int> lim(h.d);
+        std::memcpy(lim.data(), shape.data(), shape_bytes);
+        st = sg_grid_create(int(h.d), int(h.n), lim.data(), &g);
+    }
+    if (st != sg::kOk) return st;
+    if (uint64_t(g->g.N) != h.N || uint64_t(g->g.S) != h.S || uint32_t(g->g.n) != h.n) {
+        sg_grid_destroy(g);
+        return fail(sg::kInvalid, std::string("sg_file_read: level set does not match the header in ") + path);
+    }
+    if (values) std::memcpy(values, vals.data(), value_bytes);
+    *out = g;
+    return sg::kOk;
+}
+
 int sg_gather_probe_f64(const sg_grid* grid, const double* d_alpha, long long num_points,
                         unsigned long long seed, double* d_out, void* stream) {
     if (!grid || num_points < 0 || (num_points > 0 && (!d_alpha || !d_out)))
@@ -433,6 +575,22 @@ int sg_gather_probe_f32(const sg_grid* grid, const float* d_alpha, long long num
     if (st != sg::kOk) return st;
     return sg::launch_gather_probe_f32(grid->g, grid->t, d_alpha, num_points, uint32_t(seed),
                                        d_out, as_stream(stream));
+}
+
+int sg_points_to_row_major_f64(const double* d_in, int layout, int num_dims, long long num_points,
+                               double* d_out, void* stream) {
+    if (num_points < 0 || (num_points > 0 && (!d_in || !d_out)) || d_in == d_out)
+        return fail(sg::kInvalid, "sg_points_to_row_major_f64: bad argument");
+    return sg::launch_points_to_row_major_f64(d_in, layout, num_dims, num_points, d_out,
+                                              as_stream(stream));
+}
+
+int sg_points_to_row_major_f32(const float* d_in, int layout, int num_dims, long long num_points,
+                               float* d_out, void* stream) {
+    if (num_points < 0 || (num_points > 0 && (!d_in || !d_out)) || d_in == d_out)
+        return fail(sg::kInvalid, "sg_points_to_row_major_f32: bad argument");
+    return sg::launch_points_to_row_major_f32(d_in, layout, num_dims, num_points, d_out,
+                                              as_stream(stream));
 }
 
 int sg_convert_f64_to_f32(const double* d_in, float* d_out, long long count, void* stream) {

@@ -571,6 +571,25 @@ __global__ __launch_bounds__(kThreads) void k_gather_probe(const T* alpha, int64
     if (p < P) out[p] = acc;
 }
 
+// out[p][k] = the point p, coordinate k of a column-major or 32-point-tiled input; the
+// writes are coalesced, the reads strided (a one-pass transpose at memory speed).
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_points_to_row_major(const T* __restrict__ in,
+                                                                  int layout, int d, int64_t P,
+                                                                  T* __restrict__ out) {
+    const int64_t total = P * d;
+    for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < total;
+         o += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t p = o / d;
+        const int k = int(o - p * d);
+        int64_t src;
+        if (layout == kPointsColumnMajor) src = int64_t(k) * P + p;
+        else if (layout == kPointsTiled32) src = (p >> 5) * d * 32 + int64_t(k) * 32 + (p & 31);
+        else src = o;
+        out[o] = in[src];
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_f64_to_f32(const double* in, float* out,
                                                          int64_t count) {
     for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < count;
@@ -820,6 +839,26 @@ int launch_sample_f64(const Grid& g, const DeviceTables& t, const SampleParams& 
 int launch_sample_f32(const Grid& g, const DeviceTables& t, const SampleParams& sp, float* out,
                       hipStream_t s) {
     return sample(g, t, sp, out, s);
+}
+
+template <class T>
+static int points_to_row_major(const T* in, int layout, int d, int64_t P, T* out, hipStream_t s) {
+    if (layout < kPointsRowMajor || layout > kPointsTiled32 || d < 1 || d > kMaxDims || P < 0) {
+        set_error("points_to_row_major: bad layout, dimension or count");
+        return kInvalid;
+    }
+    if (P == 0) return kOk;
+    hipLaunchKernelGGL(k_points_to_row_major<T>, dim3(grid_for(P * d)), dim3(kThreads), 0, s, in,
+                       layout, d, P, out);
+    return check_launch("k_points_to_row_major");
+}
+int launch_points_to_row_major_f64(const double* in, int layout, int d, int64_t P, double* out,
+                                   hipStream_t s) {
+    return points_to_row_major(in, layout, d, P, out, s);
+}
+int launch_points_to_row_major_f32(const float* in, int layout, int d, int64_t P, float* out,
+                                   hipStream_t s) {
+    return points_to_row_major(in, layout, d, P, out, s);
 }
 
 int launch_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s) {

@@ -138,6 +138,17 @@ int launch_uniform_points_f64(int64_t count, uint64_t seed, double* out, hipStre
 int launch_uniform_points_f32(int64_t count, uint64_t seed, float* out, hipStream_t s);
 int launch_f64_to_f32(const double* in, float* out, int64_t count, hipStream_t s);
 
+// Query points in another layout -> row-major [P][d] (the layout the kernels read).
+enum PointLayout : int {
+    kPointsRowMajor = 0,      // [P][d]
+    kPointsColumnMajor = 1,   // [d][P]
+    kPointsTiled32 = 2,       // [ceil(P/32)][d][32], asg.cu:167-180
+};
+int launch_points_to_row_major_f64(const double* in, int layout, int d, int64_t P, double* out,
+                                   hipStream_t s);
+int launch_points_to_row_major_f32(const float* in, int layout, int d, int64_t P, float* out,
+                                   hipStream_t s);
+
 // Test functions sampled at the grid points on the device (values in compact order).
 enum SampleFn : int {
     kFnReference = 0,     // sum_k x_k^2 (1 - x_k)^2, asg.c:87-94, same operation order

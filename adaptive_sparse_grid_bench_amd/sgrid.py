@@ -41,6 +41,8 @@ EXPORTS = [
     "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
     "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
     "sg_gather_probe_f64", "sg_gather_probe_f32", "sg_sample_f64", "sg_sample_f32",
+    "sg_file_write", "sg_file_info", "sg_file_read",
+    "sg_points_to_row_major_f64", "sg_points_to_row_major_f32",
     "sg_batch_create", "sg_batch_destroy", "sg_batch_values", "sg_batch_set_points",
     "sg_batch_hierarchize_f64", "sg_batch_hierarchize_f32", "sg_batch_evaluate_f64",
     "sg_batch_evaluate_f32c", "sg_batch_run_graph_f64", "sg_batch_run_graph_f32c",
@@ -118,6 +120,11 @@ def _load() -> C.CDLL:
         "sg_uniform_points_f64": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_uniform_points_f32": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_convert_f64_to_f32": (i, [vp, vp, ll, vp]),
+        "sg_points_to_row_major_f64": (i, [vp, i, i, ll, vp, vp]),
+        "sg_points_to_row_major_f32": (i, [vp, i, i, ll, vp, vp]),
+        "sg_file_write": (i, [C.c_char_p, vp, vp, i]),
+        "sg_file_info": (i, [C.c_char_p, ip, ip, llp, llp, ip]),
+        "sg_file_read": (i, [C.c_char_p, C.POINTER(vp), vp, ll]),
         "sg_sample_f64": (i, [vp, i, C.POINTER(C.c_double), ll, vp, vp]),
         "sg_sample_f32": (i, [vp, i, C.POINTER(C.c_double), ll, vp, vp]),
         "sg_gather_probe_f64": (i, [vp, vp, ll, C.c_ulonglong, vp, vp]),
@@ -318,6 +325,38 @@ class Grid:
         check(fn(self._h, _ptr(alpha), _ptr(coords), P, _ptr(out), _stream(stream)))
 
 
+def save(path: str, grid: "Grid", values=None) -> None:
+    """Write a compact-grid file: the level set and, if given, the values (a float64 or
+    float32 array or CPU tensor in compact order)."""
+    if values is None:
+        check(lib.sg_file_write(os.fsencode(path), grid._h, None, 0))
+        return
+    v = np.ascontiguousarray(values.cpu().numpy() if hasattr(values, "cpu") else values)
+    if v.dtype not in (np.float64, np.float32) or v.shape != (grid.N,):
+        raise ValueError("values must be float64 or float32 of shape (N,)")
+    check(lib.sg_file_write(os.fsencode(path), grid._h, v.ctypes.data, v.itemsize))
+
+
+def file_info(path: str) -> dict:
+    d, n, es = C.c_int(), C.c_int(), C.c_int()
+    N, S = C.c_longlong(), C.c_longlong()
+    check(lib.sg_file_info(os.fsencode(path), C.byref(d), C.byref(n), C.byref(N), C.byref(S),
+                           C.byref(es)))
+    return {"d": d.value, "n": n.value, "N": N.value, "S": S.value, "elem_bytes": es.value}
+
+
+def load(path: str):
+    """Read a compact-grid file: (Grid, values as a numpy array or None)."""
+    info = file_info(path)
+    es = info["elem_bytes"]
+    vals = np.empty(info["N"], np.float64 if es == 8 else np.float32) if es else None
+    h = C.c_void_p()
+    check(lib.sg_file_read(os.fsencode(path), C.byref(h),
+                           vals.ctypes.data if vals is not None else None,
+                           vals.nbytes if vals is not None else 0))
+    return Grid(0, 0, _handle=h), vals
+
+
 def gather_probe(grid: "Grid", alpha, out, seed: int = 1, stream=None) -> None:
     """The achievable-gather pattern of grid's shape for out.numel() points (one random
     in-block gather per subspace per point, no hats): the evaluate kernels' gather
@@ -325,6 +364,18 @@ def gather_probe(grid: "Grid", alpha, out, seed: int = 1, stream=None) -> None:
     import torch
     fn = lib.sg_gather_probe_f64 if alpha.dtype == torch.float64 else lib.sg_gather_probe_f32
     check(fn(grid._h, _ptr(alpha), out.numel(), seed, _ptr(out), _stream(stream)))
+
+
+POINT_LAYOUTS = {"row_major": 0, "column_major": 1, "tiled32": 2}
+
+
+def points_to_row_major(src, layout: str, d: int, num_points: int, out, stream=None) -> None:
+    """Device copy of query points held column-major ([d][P]) or in the CUDA reference's
+    32-point tiles ([ceil(P/32)][d][32], asg.cu:167-180) into row-major out [P][d]."""
+    import torch
+    fn = lib.sg_points_to_row_major_f64 if out.dtype == torch.float64 else \
+        lib.sg_points_to_row_major_f32
+    check(fn(_ptr(src), POINT_LAYOUTS[layout], d, num_points, _ptr(out), _stream(stream)))
 
 
 def uniform_points(out, seed: int = 42, stream=None) -> None:

@@ -256,8 +256,37 @@ int sg_uniform_points_f64(long long count, unsigned long long seed, double *d_ou
                           void *stream);
 int sg_uniform_points_f32(long long count, unsigned long long seed, float *d_out,
                           void *stream);
+/* Query points held in another layout -> row-major [P][d], on the device (d_out must
+ * not overlap d_in; it holds P * num_dims values):
+ *   SG_POINTS_ROW_MAJOR     [P][d] (a copy; asg.c:1259)
+ *   SG_POINTS_COLUMN_MAJOR  [d][P]
+ *   SG_POINTS_TILED32       [ceil(P/32)][d][32], the CUDA reference's transposed layout
+ *                           (init_coord_mat_trans, asg.cu:167-180; its tail tile padded)
+ * Replaces the host transpose of the CUDA driver (asg.cu:1679). */
+#define SG_POINTS_ROW_MAJOR 0
+#define SG_POINTS_COLUMN_MAJOR 1
+#define SG_POINTS_TILED32 2
+int sg_points_to_row_major_f64(const double *d_in, int layout, int num_dims, long long num_points,
+                               double *d_out, void *stream);
+int sg_points_to_row_major_f32(const float *d_in, int layout, int num_dims, long long num_points,
+                               float *d_out, void *stream);
 /* FP64 -> FP32 rounding copy (surpluses for the FP32 evaluate). */
 int sg_convert_f64_to_f32(const double *d_in, float *d_out, long long count, void *stream);
+
+/* ------------------------------------------------------------ files --------- */
+/* A compact-grid file: a 64-byte header (magic "SGRIDv01", d, n, kind, element size,
+ * N, S, FNV-1a 64 checksum of the rest), the level set (restricted-regular: int32
+ * limits[d]; explicit: uint8 levels[S][d] in compact order) and N values in compact
+ * order, little-endian.  It lets a server load resident grids without rebuilding them.
+ * The reference rebuilds its grid every run (asg.c:1767) and has no file format.
+ * Values are host memory; elem_bytes is 8, 4, or 0 (a shape-only file). */
+int sg_file_write(const char *path, const sg_grid *grid, const void *values, int elem_bytes);
+int sg_file_info(const char *path, int *num_dims, int *num_levels, long long *num_grid_points,
+                 long long *num_subspaces, int *elem_bytes);
+/* Rebuilds the grid (restricted-regular or explicit) and, if values is not NULL, copies
+ * the values into it (values_bytes >= N * elem_bytes); the checksum is verified when
+ * values are read. */
+int sg_file_read(const char *path, sg_grid **grid, void *values, long long values_bytes);
 
 /* ------------------------------------------------------ measurement --------- */
 

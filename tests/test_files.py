@@ -1,0 +1,81 @@
+"""Compact-grid files: write, inspect and read back the level set and values.
+
+Bars: a round trip gives the same grid (N, S, subspace list, compact order) and the same
+values bit for bit, for restricted-regular and explicit level sets, FP64 and FP32; a
+file that is not a grid file, is truncated or fails its checksum is rejected with
+SG_E_INVALID, and a too-small value buffer with SG_E_RANGE.  Host-side only."""
+import os
+
+import numpy as np
+import pytest
+
+from adaptive_sparse_grid_bench_amd import workloads as W
+
+
+@pytest.fixture(scope="module")
+def sg():
+    from adaptive_sparse_grid_bench_amd import sgrid
+    return sgrid
+
+
+def same_grid(a, b):
+    assert (a.d, a.N, a.S) == (b.d, b.N, b.S)
+    la, ba = a.subspaces()
+    lb, bb = b.subspaces()
+    assert (la == lb).all() and (ba == bb).all()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("kind", ["regular", "restricted", "explicit"])
+def test_round_trip(sg, tmp_path, dtype, kind):
+    if kind == "regular":
+        g = sg.Grid(10, 5)
+    elif kind == "restricted":
+        g = sg.Grid(4, 5, [4, 1, 4, 2])
+    else:
+        g = sg.Grid.explicit(8, W.adaptive_level_set(d=8, base_n=3, extra=4, seed=3)[0])
+    v = np.random.default_rng(1).standard_normal(g.N).astype(dtype)
+    path = str(tmp_path / "g.sgrid")
+    sg.save(path, g, v)
+    info = sg.file_info(path)
+    assert info == {"d": g.d, "n": g.n, "N": g.N, "S": g.S, "elem_bytes": v.itemsize}
+    g2, v2 = sg.load(path)
+    same_grid(g, g2)
+    assert v2.dtype == dtype and np.array_equal(v2, v)
+    assert os.path.getsize(path) == 64 + (g.S * g.d if kind == "explicit" else 4 * g.d) + v.nbytes
+
+
+def test_shape_only_file(sg, tmp_path):
+    g = sg.Grid(3, 5)
+    path = str(tmp_path / "shape.sgrid")
+    sg.save(path, g)
+    g2, v2 = sg.load(path)
+    same_grid(g, g2)
+    assert v2 is None
+
+
+def test_rejects_bad_files(sg, tmp_path):
+    g = sg.Grid(3, 5)
+    v = np.arange(g.N, dtype=np.float64)
+    path = tmp_path / "g.sgrid"
+    sg.save(str(path), g, v)
+    raw = path.read_bytes()
+    cases = {
+        "magic": b"XXXXXXXX" + raw[8:],
+        "truncated": raw[:-8],
+        "checksum": raw[:-1] + bytes([raw[-1] ^ 1]),
+        "header": raw[:8] + b"\x00" * 4 + raw[12:],
+    }
+    for name, data in cases.items():
+        p = tmp_path / f"{name}.sgrid"
+        p.write_bytes(data)
+        with pytest.raises(sg.SgError) as e:
+            sg.load(str(p))
+        assert e.value.status == sg.SG_E_INVALID, name
+    with pytest.raises(sg.SgError):
+        sg.load(str(tmp_path / "missing.sgrid"))
+    import ctypes as C
+    h = C.c_void_p()
+    small = np.empty(g.N - 1)
+    st = sg.lib.sg_file_read(os.fsencode(str(path)), C.byref(h), small.ctypes.data, small.nbytes)
+    assert st == sg.SG_E_RANGE

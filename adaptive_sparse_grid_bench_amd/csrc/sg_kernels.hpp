@@ -101,7 +101,7 @@ constexpr int kStagedVariant = 3;
 // default variants by dimension and precision, from timings at 2^22-2^24 points on
 // d = 5, 10, 12, 16, 20 (the autotuner refines them per grid shape)
 inline int default_variant(int d, bool f32) {
-    if (f32) return d <= 6 ? kStagedVariant : 2;
+    if (f32) return d <= 12 ? kStagedVariant : 2;
     return d >= 16 ? 1 : kStagedVariant;
 }
 // the variant a launch runs: the staged walk falls back to variant 0 for a grid whose

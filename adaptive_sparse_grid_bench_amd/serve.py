@@ -11,11 +11,13 @@ by a closed-loop run.  Latency = completion - arrival (queueing included); throu
 points / (last completion - first arrival).
 
 Usage: python -m adaptive_sparse_grid_bench_amd.serve [--grids 64] [--requests 600]
+       [--grid-dir DIR]   (resident grids from compact-grid files, see sgrid.save/load)
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import threading
 import time
 
@@ -25,10 +27,36 @@ from . import sgrid
 from . import workloads as W
 
 
-def make_grids(count: int = 64, seed: int = 21, adaptive_every: int = 4,
-               max_points: int = 1 << 25):
-    """Resident grids with FP64 surpluses and FP32 copies (torch CUDA tensors)."""
+def load_grids(grid_dir: str):
+    """Resident grids read from compact-grid files (grid_NNN.sgrid, FP64 surpluses)."""
     import torch
+    grids = []
+    for name in sorted(os.listdir(grid_dir)):
+        if not name.endswith(".sgrid"):
+            continue
+        g, a = sgrid.load(os.path.join(grid_dir, name))
+        a = torch.from_numpy(a).cuda()
+        kind = "regular" if name.endswith("_r.sgrid") else "adaptive"
+        grids.append({"grid": g, "a64": a, "a32": a.float(), "kind": kind})
+    torch.cuda.synchronize()
+    return grids
+
+
+def save_grids(grids, grid_dir: str) -> None:
+    os.makedirs(grid_dir, exist_ok=True)
+    for i, g in enumerate(grids):
+        tag = "r" if g["kind"] == "regular" else "a"
+        sgrid.save(os.path.join(grid_dir, f"grid_{i:03d}_{tag}.sgrid"), g["grid"], g["a64"])
+
+
+def make_grids(count: int = 64, seed: int = 21, adaptive_every: int = 4,
+               max_points: int = 1 << 25, grid_dir: str | None = None):
+    """Resident grids with FP64 surpluses and FP32 copies (torch CUDA tensors).  With
+    grid_dir, grids are read from its compact-grid files when present and written there
+    after being built otherwise."""
+    import torch
+    if grid_dir and os.path.isdir(grid_dir) and any(f.endswith(".sgrid") for f in os.listdir(grid_dir)):
+        return load_grids(grid_dir)
     rng = np.random.default_rng(seed)
     grids = []
     k = 0
@@ -56,6 +84,8 @@ def make_grids(count: int = 64, seed: int = 21, adaptive_every: int = 4,
         g.hierarchize(a)
         grids.append({"grid": g, "a64": a, "a32": a.float(), "kind": kind})
     torch.cuda.synchronize()
+    if grid_dir:
+        save_grids(grids, grid_dir)
     return grids
 
 
@@ -168,9 +198,12 @@ def main():
     ap.add_argument("--grids", type=int, default=64)
     ap.add_argument("--requests", type=int, default=600)
     ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--grid-dir", default=None,
+                    help="read the resident grids from compact-grid files here (written on "
+                         "the first run)")
     args = ap.parse_args()
     t = time.time()
-    grids = make_grids(args.grids)
+    grids = make_grids(args.grids, grid_dir=args.grid_dir)
     res = experiment(grids, args.requests, args.load)
     res["setup_s"] = time.time() - t
     print(json.dumps(res), flush=True)

@@ -79,3 +79,25 @@ def test_rejects_bad_files(sg, tmp_path):
     small = np.empty(g.N - 1)
     st = sg.lib.sg_file_read(os.fsencode(str(path)), C.byref(h), small.ctypes.data, small.nbytes)
     assert st == sg.SG_E_RANGE
+
+
+@pytest.mark.gpu
+def test_loaded_grid_evaluates_identically(sg, torch_gpu, tmp_path):
+    """A grid and its surpluses read back from a file evaluate bit-identically on the GPU
+    (the served driver's resident grids come from such files)."""
+    torch = torch_gpu
+    for g in (sg.Grid(6, 6), sg.Grid.explicit(8, W.adaptive_level_set(d=8, base_n=3, extra=4,
+                                                                         seed=9)[0])):
+        a = torch.empty(g.N, dtype=torch.float64, device="cuda")
+        g.sample("gauss", a)
+        g.hierarchize(a)
+        path = str(tmp_path / "g.sgrid")
+        sg.save(path, g, a)
+        g2, a2 = sg.load(path)
+        x = torch.from_numpy(W.uniform_points(2000, g.d, seed=4)).cuda()
+        o1 = torch.empty(2000, dtype=torch.float64, device="cuda")
+        o2 = torch.empty_like(o1)
+        g.evaluate(a, x, o1)
+        g2.evaluate(torch.from_numpy(a2).cuda(), x, o2)
+        torch.cuda.synchronize()
+        assert torch.equal(o1, o2)

@@ -59,6 +59,13 @@ struct AccF64 {
     __device__ __forceinline__ void pin() { asm volatile("" : "+v"(s)); }
 };
 
+// FP64 sums a static block nested (block_consume_nested); the Kahan accumulator keeps
+// its leaf-by-leaf compensated sum.
+template <class Acc>
+inline constexpr bool kNestedSum = false;
+template <>
+inline constexpr bool kNestedSum<AccF64> = true;
+
 // Kahan-compensated FP32 accumulation across subspaces.  Contraction is off inside the
 // update, so every walk variant computes exactly the written compensation whatever the
 // compiler knows about s and c (at the first leaf both are constant zeros).
@@ -264,16 +271,57 @@ __device__ __forceinline__ void block_consume(const T (&a)[B], const Ln& ln, T p
     }
 }
 
+// FP64 blocks are summed nested instead of leaf by leaf:
+//   pp * sum_{l_{J-1}} ph_{J-1} ( ... sum_{l_1} ph_1 ( sum_{l_0} ph_0 a ) ... ).
+// s[0] runs over l_0 for fixed higher levels (consecutive leaves); when a run of level j
+// ends its sum is folded into s[j+1] with weight ph_{j+1}; the block's sum times the
+// prefix product pp goes into the accumulator.  One FMA per leaf and one per finished
+// run, where the leaf-by-leaf form recomputes part of the hat product per leaf.
+// Run boundaries are compile-time facts of the simplex enumeration.
+template <int J, int R, int c, int j>
+__host__ __device__ constexpr int run_max() {         // largest l_j given the levels above
+    int m = R;
+    for (int i = j + 1; i < J; ++i) m -= Simplex<J, R>().l[c][i];
+    return m;
+}
+
+template <int J, int R, int c, int j, class Ln, class T, class Acc>
+__device__ __forceinline__ void nested_fold(T (&sn)[J], const Ln& ln, T pp, Acc& acc) {
+    if constexpr (j == J) {
+        acc.s = fma(pp, sn[J - 1], acc.s);
+    } else {
+        constexpr int l = simplex_level<J, R, c, j>();
+        if constexpr (l == 0) sn[j] = ln.ph[j][0] * sn[j - 1];
+        else sn[j] = fma(ln.ph[j][l], sn[j - 1], sn[j]);
+        if constexpr (l == run_max<J, R, c, j>()) nested_fold<J, R, c, j + 1>(sn, ln, pp, acc);
+    }
+}
+
+template <int J, int R, int B, int c, int cend, class Ln, class T, class Acc>
+__device__ __forceinline__ void block_consume_nested(const T (&a)[B], T (&sn)[J], const Ln& ln,
+                                                     T pp, Acc& acc) {
+    if constexpr (c < cend) {
+        constexpr int l0 = simplex_level<J, R, c, 0>();
+        if constexpr (l0 == 0) sn[0] = ln.ph[0][0] * a[c % B];
+        else sn[0] = fma(ln.ph[0][l0], a[c % B], sn[0]);
+        if constexpr (l0 == run_max<J, R, c, 0>()) nested_fold<J, R, c, 1>(sn, ln, pp, acc);
+        block_consume_nested<J, R, B, c + 1, cend>(a, sn, ln, pp, acc);
+    }
+}
+
 template <int J, int R, int B, bool LDS, int c0, class Ln, class T, class St>
 __device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, int hsum,
-                                              St& st) {
+                                              St& st, T (&sn)[J]) {
     constexpr int C = Simplex<J, R>::C;
     if constexpr (c0 < C) {
         constexpr int cend = (c0 + B < C) ? c0 + B : C;
         T a[B];
         block_issue<J, R, B, LDS, c0, cend>(a, ln, off, hsum, st);
-        block_consume<J, R, B, c0, cend>(a, ln, pp, st.acc);
-        block_batches<J, R, B, LDS, cend>(ln, pp, off, hsum, st);
+        if constexpr (kNestedSum<decltype(st.acc)>)
+            block_consume_nested<J, R, B, c0, cend>(a, sn, ln, pp, st.acc);
+        else
+            block_consume<J, R, B, c0, cend>(a, ln, pp, st.acc);
+        block_batches<J, R, B, LDS, cend>(ln, pp, off, hsum, st, sn);
     }
 }
 
@@ -284,16 +332,17 @@ __device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, 
 template <int J, int R, int B, class Ln, class T, class St>
 __device__ __forceinline__ void static_block(const Ln& ln, T pp, uint32_t off, int hsum,
                                              St& st) {
+    T sn[J];                   // nested partial sums (FP64 blocks)
     if constexpr (St::kStaged) {
         constexpr int kHmax = stage_hmax(J, R, int(sizeof(T)));
         if (hsum <= kHmax) {
             st.block_start(st.q);
-            block_batches<J, R, B, true, 0>(ln, pp, off, hsum, st);
+            block_batches<J, R, B, true, 0>(ln, pp, off, hsum, st, sn);
         } else {
-            block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st);
+            block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st, sn);
         }
     } else {
-        block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st);
+        block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st, sn);
     }
     st.q += Simplex<J, R>::C;
 }

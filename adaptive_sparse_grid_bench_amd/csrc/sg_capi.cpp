@@ -476,7 +476,7 @@ int sg_file_write(const char* path, const sg_grid* grid, const void* values, int
     h.elem_bytes = uint32_t(elem_bytes);
     h.N = uint64_t(g.N);
     h.S = uint64_t(g.S);
-    std::vector<int32_t> lim(g.limits, g.limits + g.d);
+    const std::vector<int32_t> lim(g.limits, g.limits + g.d);   // d <= 20 entries
     const void* shape = g.explicit_set ? static_cast<const void*>(g.sub_levels.data())
                                        : static_cast<const void*>(lim.data());
     const size_t shape_bytes = g.explicit_set ? g.sub_levels.size() : lim.size() * 4;
@@ -527,8 +527,20 @@ int sg_file_read(const char* path, sg_grid** out, void* values, long long values
     const size_t value_bytes = size_t(h.N) * h.elem_bytes;
     if (values && values_bytes < (long long)value_bytes)
         return fail(sg::kRange, "sg_file_read: value buffer smaller than the file's values");
-    std::vector<unsigned char> shape(shape_bytes);
-    std::vector<unsigned char> vals(values ? value_bytes : 0);
+    // the file must hold what the header announces before anything is allocated for it
+    if (std::fseek(f, 0, SEEK_END) != 0)
+        return fail(sg::kInvalid, std::string("sg_file_read: cannot seek in ") + path);
+    const long end = std::ftell(f);
+    if (end < 0 || uint64_t(end) < sizeof(FileHeader) + shape_bytes + value_bytes ||
+        std::fseek(f, long(sizeof(FileHeader)), SEEK_SET) != 0)
+        return fail(sg::kInvalid, std::string("sg_file_read: truncated file ") + path);
+    std::vector<unsigned char> shape, vals;
+    try {
+        shape.resize(shape_bytes);
+        vals.resize(values ? value_bytes : 0);
+    } catch (const std::bad_alloc&) {
+        return fail(sg::kNoMem, "sg_file_read: host allocation failed");
+    }
     if (std::fread(shape.data(), 1, shape_bytes, f) != shape_bytes)
         return fail(sg::kInvalid, std::string("sg_file_read: truncated level set in ") + path);
     uint64_t c = fnv1a(0xcbf29ce484222325ull, shape.data(), shape_bytes);

@@ -65,6 +65,8 @@ def test_rejects_bad_files(sg, tmp_path):
         "truncated": raw[:-8],
         "checksum": raw[:-1] + bytes([raw[-1] ^ 1]),
         "header": raw[:8] + b"\x00" * 4 + raw[12:],
+        # a header announcing 2^31 - 1 values: rejected before anything is allocated
+        "huge": raw[:32] + (2 ** 31 - 1).to_bytes(8, "little") + raw[40:],
     }
     for name, data in cases.items():
         p = tmp_path / f"{name}.sgrid"

@@ -59,14 +59,19 @@ int Batch::build(std::vector<GridRef> members) {
     }
     int maxd = 0;
     for (auto& r : grids) maxd = std::max(maxd, r.g->d);
-    // concatenated pole-family tables, one set per dimension sweep
-    hier.assign(maxd, HierDim());
-    for (int k = 0; k < maxd; ++k) {
+    // Sweeps in launch order.  For each dimension pair (k, k+1): one plane-family sweep
+    // over the grids whose pair tables exist, then pole-family sweeps of k and of k+1 over
+    // the others (grids are independent, so only each grid's own order matters).
+    hier.clear();
+    auto uses_pair = [&](const Grid& g, int k) {
+        return k % 2 == 0 && k + 1 < g.d && g.hier2[k / 2].ok;
+    };
+    auto add_pole_sweep = [&](int k) -> int {
         std::vector<int32_t> tiles, fams, blocks;
         int tv = 0, mf = 0;
         for (int i = 0; i < G; ++i) {
             const Grid& g = *grids[i].g;
-            if (k >= g.d) continue;
+            if (k >= g.d || uses_pair(g, k - k % 2)) continue;
             const HierTables& h = g.hier[k];
             const int32_t fam0 = int32_t(fams.size() / 3), blk0 = int32_t(blocks.size());
             for (size_t t = 0; t < h.tiles.size(); t += 3) {
@@ -83,7 +88,8 @@ int Batch::build(std::vector<GridRef> members) {
             for (int32_t b : h.blocks) blocks.push_back(int32_t(b + value_off[i]));
             mf = std::max(mf, h.max_families_per_tile);
         }
-        HierDim& hd = hier[k];
+        if (tiles.empty()) return kOk;
+        HierDim hd;
         hd.num_tiles = int(tiles.size() / 3);
         hd.tile_values = tv;
         hd.max_families = mf;
@@ -91,6 +97,57 @@ int Batch::build(std::vector<GridRef> members) {
         if ((st = upload_vec(hd.tiles, tiles)) != kOk) return st;
         if ((st = upload_vec(hd.fams, fams)) != kOk) return st;
         if ((st = upload_vec(hd.blocks, blocks)) != kOk) return st;
+        hier.push_back(hd);
+        return kOk;
+    };
+    auto add_pair_sweep = [&](int k) -> int {
+        std::vector<int32_t> tiles, fams, subs, blocks;
+        int tv = 0, mf = 0, ms = 0;
+        for (int i = 0; i < G; ++i) {
+            const Grid& g = *grids[i].g;
+            if (!uses_pair(g, k)) continue;
+            const PairTables& h = g.hier2[k / 2];
+            const int32_t fam0 = int32_t(fams.size() / 3), sub0 = int32_t(subs.size() / 2),
+                          blk0 = int32_t(blocks.size());
+            for (size_t t = 0; t < h.tiles.size(); t += 3) {
+                tiles.push_back(h.tiles[t] + fam0);
+                tiles.push_back(h.tiles[t + 1]);
+                tiles.push_back(h.tiles[t + 2]);
+                tv = std::max(tv, int(h.tiles[t + 2]));
+            }
+            for (size_t f = 0; f < h.fams.size(); f += 3) {
+                fams.push_back(h.fams[f]);
+                fams.push_back(h.fams[f + 1]);
+                fams.push_back(h.fams[f + 2] + sub0);
+            }
+            for (size_t q = 0; q < h.subs.size(); q += 2) {
+                subs.push_back(h.subs[q]);
+                subs.push_back(h.subs[q + 1] + blk0);
+            }
+            for (int32_t b : h.blocks) blocks.push_back(int32_t(b + value_off[i]));
+            mf = std::max(mf, h.max_families_per_tile);
+            ms = std::max(ms, h.max_subs_per_tile);
+        }
+        if (tiles.empty()) return kOk;
+        HierDim hd;
+        hd.pair = true;
+        hd.num_tiles = int(tiles.size() / 3);
+        hd.tile_values = tv;
+        hd.max_families = mf;
+        hd.max_subs = ms;
+        int st;
+        if ((st = upload_vec(hd.tiles, tiles)) != kOk) return st;
+        if ((st = upload_vec(hd.fams, fams)) != kOk) return st;
+        if ((st = upload_vec(hd.subs, subs)) != kOk) return st;
+        if ((st = upload_vec(hd.blocks, blocks)) != kOk) return st;
+        hier.push_back(hd);
+        return kOk;
+    };
+    for (int k = 0; k < maxd; k += 2) {
+        int st;
+        if ((st = add_pair_sweep(k)) != kOk) return st;
+        if ((st = add_pole_sweep(k)) != kOk) return st;
+        if (k + 1 < maxd && (st = add_pole_sweep(k + 1)) != kOk) return st;
     }
     return kOk;
 }
@@ -151,8 +208,11 @@ template <class T>
 int Batch::hierarchize(T* v, hipStream_t s) const {
     for (const HierDim& hd : hier) {
         if (hd.num_tiles == 0) continue;
-        int st = launch_hierarchize_tiles<T>(v, hd.tiles, hd.fams, hd.blocks, hd.num_tiles,
-                                             hd.tile_values, hd.max_families, s);
+        int st = hd.pair
+            ? launch_hierarchize_pairs<T>(v, hd.tiles, hd.fams, hd.subs, hd.blocks, hd.num_tiles,
+                                          hd.tile_values, hd.max_families, hd.max_subs, s)
+            : launch_hierarchize_tiles<T>(v, hd.tiles, hd.fams, hd.blocks, hd.num_tiles,
+                                          hd.tile_values, hd.max_families, s);
         if (st != kOk) return st;
     }
     return kOk;
@@ -266,6 +326,7 @@ void Batch::release() {
     for (HierDim& hd : hier) {
         free_ptr(hd.tiles);
         free_ptr(hd.fams);
+        free_ptr(hd.subs);
         free_ptr(hd.blocks);
     }
     free_ptr(d_desc);

@@ -21,13 +21,16 @@ struct Batch {
         int variant_f64;        // the grid's tuned walk variants
         int variant_f32;
     };
-    struct HierDim {
+    struct HierDim {            // one launch of the hierarchization
+        bool pair = false;      // plane families of (k, k+1), else pole families of k
         int32_t* tiles = nullptr;
         int32_t* fams = nullptr;
+        int32_t* subs = nullptr;
         int32_t* blocks = nullptr;
         int num_tiles = 0;
         int tile_values = 0;
         int max_families = 0;
+        int max_subs = 0;
     };
     struct Group {               // one evaluation launch
         bool explicit_set;
@@ -44,7 +47,7 @@ struct Batch {
     std::vector<int64_t> value_off;   // [G+1]: grid g's surpluses start here
     std::vector<int64_t> point_off;   // [G+1]: grid g's points / outputs start here
     std::vector<int64_t> coord_off;   // [G+1]: grid g's coordinates start here (elements)
-    std::vector<HierDim> hier;        // one per dimension sweep
+    std::vector<HierDim> hier;        // the sweeps, in launch order
     std::vector<Group> groups;
     eval::GridDesc* d_desc = nullptr;
     eval::WorkItem* d_work = nullptr;

@@ -535,6 +535,81 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
         }
         close_tile();
     }
+    // hierarchization: plane families of the dimension pairs (2p, 2p+1)
+    g.hier2.assign(d / 2, PairTables());
+    for (int p = 0; p < d / 2; ++p) {
+        const int k = 2 * p;
+        PairTables& h = g.hier2[p];
+        const int tile_target = std::min(std::max(hier_tile_values, 64), kPairMaxValues);
+        int32_t tile_first = 0, tile_fams = 0, tile_vals = 0, tile_subs = 0;
+        auto close_tile = [&]() {
+            if (tile_fams == 0) return;
+            h.tiles.push_back(tile_first);
+            h.tiles.push_back(tile_fams);
+            h.tiles.push_back(tile_vals);
+            h.max_families_per_tile = std::max(h.max_families_per_tile, int(tile_fams));
+            h.max_subs_per_tile = std::max(h.max_subs_per_tile, int(tile_subs));
+            h.max_tile_values = std::max(h.max_tile_values, int(tile_vals));
+            tile_first += tile_fams;
+            tile_fams = 0;
+            tile_vals = 0;
+            tile_subs = 0;
+        };
+        bool fits = true;
+        for (int64_t id = 0; id < g.S && fits; ++id) {
+            const uint8_t* l = &g.sub_levels[id * d];
+            if (l[k] != 0 || l[k + 1] != 0) continue;     // families rooted at (0, 0)
+            int c = 0, Bfix = 0;
+            for (int j = 0; j < d; ++j) {
+                lv[j] = l[j];
+                if (j != k && j != k + 1) c += l[j];
+                if (j > k + 1) Bfix += l[j];
+            }
+            std::vector<int32_t> sub_first, sub_block, bases;
+            int64_t vals = 0;
+            int nblocks = 0;
+            for (int a = 0;; ++a) {
+                lv[k] = a;
+                lv[k + 1] = 0;
+                if (subspace_id(g, lv) < 0) break;
+                sub_first.push_back(int32_t(vals));
+                sub_block.push_back(int32_t(bases.size()));
+                int mb = -1;
+                for (int b = 0;; ++b) {
+                    lv[k + 1] = b;
+                    const int64_t sid = subspace_id(g, lv);
+                    if (sid < 0) break;
+                    bases.push_back(g.sub_base[sid]);
+                    mb = b;
+                }
+                vals += (int64_t(1) << (c + a)) * ((int64_t(2) << mb) - 1);
+                nblocks += mb + 1;
+                if (vals > kPairMaxValues) break;
+            }
+            lv[k] = 0;
+            lv[k + 1] = 0;
+            if (vals > kPairMaxValues) {
+                fits = false;
+                break;
+            }
+            if (nblocks == 1) continue;                    // one subspace: no parents
+            if (tile_fams > 0 && tile_vals + vals > tile_target) close_tile();
+            h.fams.push_back(tile_vals);
+            h.fams.push_back(c | (Bfix << 8) | ((int(sub_first.size()) - 1) << 16));
+            h.fams.push_back(int32_t(h.subs.size() / 2));
+            for (size_t a = 0; a < sub_first.size(); ++a) {
+                h.subs.push_back(sub_first[a]);
+                h.subs.push_back(int32_t(h.blocks.size()) + sub_block[a]);
+            }
+            h.blocks.insert(h.blocks.end(), bases.begin(), bases.end());
+            tile_vals += int32_t(vals);
+            tile_subs += int32_t(sub_first.size());
+            ++tile_fams;
+        }
+        close_tile();
+        h.ok = fits;
+        if (!fits) h = PairTables();
+    }
     return kOk;
 }
 

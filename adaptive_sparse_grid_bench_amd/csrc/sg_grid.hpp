@@ -82,6 +82,25 @@ struct HierTables {
     int max_families_per_tile = 0;
 };
 
+// Plane families of the dimension pair (k, k+1): the subspaces that share every level
+// but l_k and l_{k+1}.  A family is stored as its dimension-(k+1) poles for l_k = a =
+// 0..A, one after another ("sub-poles"); sub-pole a holds blocks b = 0..mb(a), block b
+// at 2^(c+a) (2^b - 1) within it, c the family's level sum outside k and k+1.  Both
+// dimensions' parents of every point lie in its family, so one workgroup sweeps k and
+// then k+1 in LDS.
+constexpr int kPairMaxValues = 4096;  // largest tile (and family) of the pair sweep
+struct PairTables {
+    bool ok = false;                  // every family fits kPairMaxValues
+    std::vector<int32_t> tiles;       // [T][3]: first family, family count, value count
+    std::vector<int32_t> fams;        // [F][3]: first value in the tile, c|Bfix<<8|A<<16,
+                                      //          first sub-pole
+    std::vector<int32_t> subs;        // [P][2]: first value in the family, first block
+    std::vector<int32_t> blocks;      // base offset of each block
+    int max_families_per_tile = 0;
+    int max_subs_per_tile = 0;
+    int max_tile_values = 0;
+};
+
 // How the staged walk copies one grid's surplus blocks into LDS, for one element size.
 // The whole static blocks (stage_whole) are packed in walk order into stages; stage k
 // lives in LDS buffer k % 2 and holds whole blocks only, so the kernel checks for a stage
@@ -114,6 +133,7 @@ struct Grid {
     std::vector<int32_t> dfs_base;    // [S] base offset per leaf, in walk order
     std::vector<uint8_t> dfs_bound;   // explicit sets: per walk node, the largest l_k
     std::vector<HierTables> hier;     // [d]
+    std::vector<PairTables> hier2;    // [d/2]: dimensions (2p, 2p+1)
     StagePlan stage[2];               // [0]: 8-byte surpluses, [1]: 4-byte
 };
 

@@ -347,14 +347,212 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_dim(T* __restrict__ v,
     }
 }
 
+// One workgroup per tile of plane families of the dimension pair (k, k+1): the tile is
+// loaded into LDS, dimension k is swept (every value from the values as loaded), then
+// dimension k+1 (from the values after the k sweep), and the tile is written back once.
+// Each thread keeps the new values of its PER points in registers between reading and
+// writing, so every parent is read before its own update, as in asg.c:557-571.  The
+// tile's family and sub-pole tables sit in LDS.  Point layout: see PairTables.
+template <class T, int PER>
+__global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v,
+                                                               const int32_t* __restrict__ tiles,
+                                                               const int32_t* __restrict__ fams,
+                                                               const int32_t* __restrict__ subs,
+                                                               const int32_t* __restrict__ blocks,
+                                                               int lds_values, int lds_fams,
+                                                               int lds_subs) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    T* buf = reinterpret_cast<T*>(smem);
+    int32_t* fst = reinterpret_cast<int32_t*>(smem + size_t(lds_values) * sizeof(T));
+    int32_t* fmeta = fst + lds_fams;
+    int32_t* fsub = fmeta + lds_fams;
+    int32_t* sfirst = fsub + lds_fams;          // sub-pole starts within their family
+    int32_t* sblock = sfirst + lds_subs;        // sub-pole first block
+    const int f0 = tiles[3 * blockIdx.x];
+    const int nf = tiles[3 * blockIdx.x + 1];
+    const int nv = tiles[3 * blockIdx.x + 2];
+    const int s0 = fams[3 * f0 + 2];
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+        fst[i] = fams[3 * (f0 + i)];
+        fmeta[i] = fams[3 * (f0 + i) + 1];
+        fsub[i] = fams[3 * (f0 + i) + 2] - s0;
+    }
+    // the tile's sub-poles: from the first family's to the last family's last one
+    const int last_meta = fams[3 * (f0 + nf - 1) + 1];
+    const int nsub = fams[3 * (f0 + nf - 1) + 2] - s0 + (last_meta >> 16) + 1;
+    for (int i = threadIdx.x; i < nsub; i += blockDim.x) {
+        sfirst[i] = subs[2 * (s0 + i)];
+        sblock[i] = subs[2 * (s0 + i) + 1];
+    }
+    __syncthreads();
+
+    int pf[PER], pa[PER], pb[PER], pr[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int val = threadIdx.x + i * kThreads;
+        pf[i] = pa[i] = pb[i] = pr[i] = 0;
+        if (val < nv) {
+            int lo = 0, hi = nf - 1;            // last family whose start is <= val
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (fst[mid] <= val) lo = mid;
+                else hi = mid - 1;
+            }
+            const int meta = fmeta[lo];
+            const int A = meta >> 16;
+            const int sf = fsub[lo];
+            const int local = val - fst[lo];
+            int a = 0;
+            while (a < A && sfirst[sf + a + 1] <= local) ++a;
+            const int sl = local - sfirst[sf + a];
+            const int cc = (meta & 255) + a;
+            const int b = 31 - __clz((sl >> cc) + 1);
+            pf[i] = lo;
+            pa[i] = a;
+            pb[i] = b;
+            pr[i] = sl - (((1 << b) - 1) << cc);
+            buf[val] = v[blocks[sblock[sf + a] + b] + pr[i]];
+        }
+    }
+    __syncthreads();
+
+    T nv_[PER];
+    // dimension k: parent (pl, b) in sub-pole pl, in-block index hi | pi | i_{k+1} lo
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int val = threadIdx.x + i * kThreads;
+        nv_[i] = T(0);
+        if (val < nv) {
+            T x = buf[val];
+            const int a = pa[i], b = pb[i], r = pr[i];
+            if (a > 0) {
+                const int meta = fmeta[pf[i]];
+                const int c = meta & 255, Bfix = (meta >> 8) & 255;
+                const int fb = fst[pf[i]], sf = fsub[pf[i]];
+                const int sh = b + Bfix;
+                const int midlo = r & ((1 << sh) - 1);
+                const int ik = (r >> sh) & ((1 << a) - 1);
+                const int hi = r >> (sh + a);
+                T left = T(0), right = T(0);
+                if (ik != 0) {                              // asg.c:828-840
+                    const int tz = __builtin_ctz(ik);
+                    const int pl = a - 1 - tz;
+                    const int pi = ik >> (tz + 1);
+                    left = buf[fb + sfirst[sf + pl] + (((1 << b) - 1) << (c + pl)) +
+                               (hi << (sh + pl)) + (pi << sh) + midlo];
+                }
+                if (ik != (1 << a) - 1) {                   // asg.c:842-854
+                    const int tz = __builtin_ctz(ik + 1);
+                    const int pl = a - 1 - tz;
+                    const int pi = (ik + 1) >> (tz + 1);
+                    right = buf[fb + sfirst[sf + pl] + (((1 << b) - 1) << (c + pl)) +
+                                (hi << (sh + pl)) + (pi << sh) + midlo];
+                }
+                x = x - (left + right) * T(0.5);
+            }
+            nv_[i] = x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int val = threadIdx.x + i * kThreads;
+        if (val < nv) buf[val] = nv_[i];
+    }
+    __syncthreads();
+    // dimension k+1: within the sub-pole, as the one-dimension sweep
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int val = threadIdx.x + i * kThreads;
+        if (val < nv) {
+            T x = buf[val];
+            const int a = pa[i], b = pb[i], r = pr[i];
+            const int sf = fsub[pf[i]];
+            if (b > 0) {
+                const int meta = fmeta[pf[i]];
+                const int Bfix = (meta >> 8) & 255;
+                const int cc = (meta & 255) + a;
+                const int base = fst[pf[i]] + sfirst[sf + a];
+                const int lo = r & ((1 << Bfix) - 1);
+                const int ik = (r >> Bfix) & ((1 << b) - 1);
+                const int hi = r >> (Bfix + b);
+                T left = T(0), right = T(0);
+                if (ik != 0) {
+                    const int tz = __builtin_ctz(ik);
+                    const int pl = b - 1 - tz;
+                    const int pi = ik >> (tz + 1);
+                    left = buf[base + (((1 << pl) - 1) << cc) + (hi << (Bfix + pl)) +
+                               (pi << Bfix) + lo];
+                }
+                if (ik != (1 << b) - 1) {
+                    const int tz = __builtin_ctz(ik + 1);
+                    const int pl = b - 1 - tz;
+                    const int pi = (ik + 1) >> (tz + 1);
+                    right = buf[base + (((1 << pl) - 1) << cc) + (hi << (Bfix + pl)) +
+                                (pi << Bfix) + lo];
+                }
+                x = x - (left + right) * T(0.5);
+            }
+            v[blocks[sblock[sf + a] + b] + r] = x;
+        }
+    }
+}
+
+}  // namespace
+
+template <class T>
+int launch_hierarchize_pairs(T* v, const int32_t* tiles, const int32_t* fams, const int32_t* subs,
+                             const int32_t* blocks, int num_tiles, int tile_values,
+                             int max_families, int max_subs, hipStream_t s) {
+    if (tile_values > kPairMaxValues) {
+        set_error("hierarchize: plane-family tile larger than the kernel's limit");
+        return kUnsupported;
+    }
+    // LDS: the largest tile's values, family and sub-pole tables
+    const size_t lds = size_t(tile_values) * sizeof(T) + size_t(max_families) * 12 +
+                       size_t(max_subs) * 8;
+    const int per = (tile_values + kThreads - 1) / kThreads;
+#define SG_PAIR(PER)                                                                        \
+    hipLaunchKernelGGL((k_hierarchize_pair<T, PER>), dim3(num_tiles), dim3(kThreads), lds, s, \
+                       v, tiles, fams, subs, blocks, tile_values, max_families, max_subs)
+    if (per <= 2) SG_PAIR(2);
+    else if (per <= 4) SG_PAIR(4);
+    else if (per <= 8) SG_PAIR(8);
+    else SG_PAIR(16);
+#undef SG_PAIR
+    return check_launch("k_hierarchize_pair");
+}
+template int launch_hierarchize_pairs<double>(double*, const int32_t*, const int32_t*,
+                                              const int32_t*, const int32_t*, int, int, int, int,
+                                              hipStream_t);
+template int launch_hierarchize_pairs<float>(float*, const int32_t*, const int32_t*,
+                                             const int32_t*, const int32_t*, int, int, int, int,
+                                             hipStream_t);
+
+namespace {
+
 template <class T>
 int hierarchize_impl(const Grid& g, const DeviceTables& t, T* v, hipStream_t s) {
-    for (int k = 0; k < g.d; ++k) {
+    for (int k = 0; k < g.d;) {
+        const bool pair = (k % 2 == 0) && k + 1 < g.d && t.pairs[k / 2].ok;
+        if (pair) {
+            const DeviceTables::Pair& pr = t.pairs[k / 2];
+            if (pr.num_tiles > 0) {
+                int st = launch_hierarchize_pairs<T>(v, pr.tiles, pr.fams, pr.subs, pr.blocks,
+                                                     pr.num_tiles, pr.tile_values,
+                                                     pr.max_families, pr.max_subs, s);
+                if (st != kOk) return st;
+            }
+            k += 2;
+            continue;
+        }
         const DeviceTables::Dim& dm = t.dims[k];
-        if (dm.num_tiles == 0) continue;
-        int st = launch_hierarchize_tiles<T>(v, dm.tiles, dm.fams, dm.blocks, dm.num_tiles,
-                                             dm.tile_values, dm.max_families, s);
-        if (st != kOk) return st;
+        if (dm.num_tiles > 0) {
+            int st = launch_hierarchize_tiles<T>(v, dm.tiles, dm.fams, dm.blocks, dm.num_tiles,
+                                                 dm.tile_values, dm.max_families, s);
+            if (st != kOk) return st;
+        }
+        ++k;
     }
     return kOk;
 }
@@ -698,6 +896,20 @@ int upload_tables(const Grid& g, DeviceTables& t) {
         if ((st = upload(dm.fams, h.fams.data(), h.fams.size())) != kOk) return st;
         if ((st = upload(dm.blocks, h.blocks.data(), h.blocks.size())) != kOk) return st;
     }
+    for (size_t p = 0; p < g.hier2.size(); ++p) {
+        const PairTables& h = g.hier2[p];
+        DeviceTables::Pair& pr = t.pairs[p];
+        pr.ok = h.ok;
+        if (!h.ok) continue;
+        pr.num_tiles = int(h.tiles.size() / 3);
+        pr.max_families = h.max_families_per_tile;
+        pr.max_subs = h.max_subs_per_tile;
+        pr.tile_values = h.max_tile_values;
+        if ((st = upload(pr.tiles, h.tiles.data(), h.tiles.size())) != kOk) return st;
+        if ((st = upload(pr.fams, h.fams.data(), h.fams.size())) != kOk) return st;
+        if ((st = upload(pr.subs, h.subs.data(), h.subs.size())) != kOk) return st;
+        if ((st = upload(pr.blocks, h.blocks.data(), h.blocks.size())) != kOk) return st;
+    }
     return kOk;
 }
 
@@ -723,6 +935,14 @@ void free_tables(DeviceTables& t) {
         free_dev(dm.fams);
         free_dev(dm.blocks);
         dm.num_tiles = 0;
+    }
+    for (auto& pr : t.pairs) {
+        free_dev(pr.tiles);
+        free_dev(pr.fams);
+        free_dev(pr.subs);
+        free_dev(pr.blocks);
+        pr.num_tiles = 0;
+        pr.ok = false;
     }
     if (switched) (void)hipSetDevice(cur);
 }

@@ -69,6 +69,17 @@ struct DeviceTables {
         int max_families = 0;
         int tile_values = 0;
     } dims[kMaxDims];
+    struct Pair {                   // plane families of dimensions (2p, 2p+1)
+        bool ok = false;
+        int32_t* tiles = nullptr;
+        int32_t* fams = nullptr;
+        int32_t* subs = nullptr;
+        int32_t* blocks = nullptr;
+        int num_tiles = 0;
+        int max_families = 0;
+        int max_subs = 0;
+        int tile_values = 0;
+    } pairs[kMaxDims / 2];
     int hier_tile_values = 0;
 };
 
@@ -88,6 +99,11 @@ int launch_hierarchize_tiles(T* v, const int32_t* tiles, const int32_t* fams,
                              const int32_t* blocks, int num_tiles, int tile_values,
                              int max_families, hipStream_t s);
 int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipStream_t s);
+// sweep of the dimension pair (k, k+1) over a set of plane-family tiles (see PairTables)
+template <class T>
+int launch_hierarchize_pairs(T* v, const int32_t* tiles, const int32_t* fams, const int32_t* subs,
+                             const int32_t* blocks, int num_tiles, int tile_values,
+                             int max_families, int max_subs, hipStream_t s);
 
 struct EvalTuning {
     int block = 256;            // threads per workgroup

@@ -89,6 +89,47 @@ int evaluate_gpu(const char* what, float* alpha, float* coord_mat, float* out, i
     return 0;
 }
 
+// FP64 twins: host values in, host values out, through the same device buffers
+int values_f64_gpu(const char* what, double* values, bool sample, bool hierarchize) {
+    if (check_grid(what)) return -1;
+    const size_t bytes = size_t(num_grid_points) * sizeof(double);
+    double* d = static_cast<double*>(d_vals.get(bytes));
+    if (!d) return report(what, "device allocation failed");
+    if (sample) {
+        if (sg_sample_f64(g_grid, SG_FN_REFERENCE, nullptr, 0, d, nullptr) != SG_OK)
+            return report(what, sg_last_error());
+    } else if (hipMemcpy(d, values, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        return report(what, "copy to device failed");
+    }
+    if (hierarchize && sg_hierarchize_f64(g_grid, d, nullptr) != SG_OK)
+        return report(what, sg_last_error());
+    if (hipMemcpy(values, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return report(what, "copy to host failed");
+    return 0;
+}
+
+int evaluate_f64_gpu(const char* what, double* alpha, double* coord_mat, double* out, int P,
+                     bool walk) {
+    if (check_grid(what)) return -1;
+    if (P <= 0) return 0;
+    const size_t abytes = size_t(num_grid_points) * sizeof(double);
+    const size_t cbytes = size_t(P) * num_dims * sizeof(double);
+    const size_t obytes = size_t(P) * sizeof(double);
+    double* da = static_cast<double*>(d_vals.get(abytes));
+    double* dc = static_cast<double*>(d_coords.get(cbytes));
+    double* dout = static_cast<double*>(d_out.get(obytes));
+    if (!da || !dc || !dout) return report(what, "device allocation failed");
+    if (hipMemcpy(da, alpha, abytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dc, coord_mat, cbytes, hipMemcpyHostToDevice) != hipSuccess)
+        return report(what, "copy to device failed");
+    const int st = walk ? sg_evaluate_f64(g_grid, da, dc, P, dout, nullptr)
+                        : sg_evaluate_ref_f64(g_grid, da, dc, P, dout, nullptr);
+    if (st != SG_OK) return report(what, sg_last_error());
+    if (hipMemcpy(out, dout, obytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return report(what, "copy to host failed");
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -213,6 +254,15 @@ int omp_evaluate2(float* a, float* c, float* o, int P) {
 }
 int omp_evaluate5(float* a, float* c, float* o, int P) {
     return evaluate_gpu("omp_evaluate5", a, c, o, P);
+}
+
+int init_values_f64(double* values) { return values_f64_gpu("init_values_f64", values, true, false); }
+int hierarchize_f64(double* values) { return values_f64_gpu("hierarchize_f64", values, false, true); }
+int evaluate_f64(double* a, double* c, double* o, int P) {
+    return evaluate_f64_gpu("evaluate_f64", a, c, o, P, false);
+}
+int evaluate_walk_f64(double* a, double* c, double* o, int P) {
+    return evaluate_f64_gpu("evaluate_walk_f64", a, c, o, P, true);
 }
 
 }  // extern "C"

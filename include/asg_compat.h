@@ -61,6 +61,18 @@ int evaluate5(float *sg1d, float *coord_mat, float *out, int num_evals);
 int omp_evaluate2(float *sg1d, float *coord_mat, float *out, int num_evals); /* asg.c:1691 */
 int omp_evaluate5(float *sg1d, float *coord_mat, float *out, int num_evals); /* asg.c:1708 */
 
+/* FP64 twins on the grid init() built (the reference computes in FP32 only):
+ *   init_values_f64     values[N] = fct at the grid points in FP64 (asg.c:87-94, 143-152)
+ *   hierarchize_f64     values[N] in place; hierarchize0 (asg.c:557-571) in FP64, exactly
+ *   evaluate_f64        evaluate0's operation order (asg.c:1237-1274) in FP64, exactly
+ *   evaluate_walk_f64   the depth-first walk: within 1e-12 of evaluate_f64 relative to
+ *                       the sum of |terms|, much faster
+ * Same argument layouts and return values as the FP32 functions. */
+int init_values_f64(double *values);
+int hierarchize_f64(double *values);
+int evaluate_f64(double *values, double *coord_mat, double *out, int num_evals);
+int evaluate_walk_f64(double *values, double *coord_mat, double *out, int num_evals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,12 @@ class Compat:
     """One process-global instance of the interface, re-initialised per shape."""
 
     def __init__(self):
+        # torch ships a HIP runtime under the same soname as /opt/rocm's; as in sgrid's
+        # loader, torch's must be the one the process uses when both are present
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         self.lib = C.CDLL(LIB)
         lib = self.lib
         lib.init.restype = None

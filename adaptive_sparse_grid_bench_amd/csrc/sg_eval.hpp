@@ -128,6 +128,10 @@ __device__ __forceinline__ cptr<V> as_const(const V* p) { return (cptr<V>)p; }
 
 using lds_byte = __attribute__((address_space(3))) uint8_t;
 
+// s_waitcnt immediate of the gfx9 family for vmcnt(0) alone: vmcnt bits 3:0 and 15:14
+// zero, expcnt (6:4) and lgkmcnt (11:8) at their maxima
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 template <class T, class Acc, bool STG>
 struct State {
     static constexpr bool kStaged = STG;
@@ -143,18 +147,35 @@ struct State {
     int stage, num_stages;
     int next_first;             // first leaf of the next stage
 
-    // copy stage k's blocks into LDS: each wave takes a contiguous quarter of the
-    // stage's segments, reads their descriptors four at a time and moves 16 bytes per
-    // lane (the descriptor table ends in four empty entries)
+    // copy stage k's blocks into LDS, 16 bytes per lane and segment: each wave takes a
+    // contiguous quarter of the stage's full segments (64 lanes, no lane mask) and of its
+    // partial ones, and reads their descriptors four at a time (the descriptor table
+    // ends in four empty entries)
     __device__ __forceinline__ void issue(int k) {
-        const int s0 = seg_begin[k], s1 = seg_begin[k + 1];
+        const int s0 = seg_begin[2 * k], sp = seg_begin[2 * k + 1], s1 = seg_begin[2 * k + 2];
         constexpr int kWaves = kThreads / 64;      // staged launches use kThreads
         const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-        const int per = (s1 - s0 + kWaves - 1) / kWaves;
-        const int g0 = s0 + wave * per;
-        const int g1 = min(s1, g0 + per);
         const uint32_t lane = threadIdx.x & 63u;
-        for (int g = g0; g < g1; g += 4) {
+        int per = (sp - s0 + kWaves - 1) / kWaves;
+        int g0 = s0 + wave * per;
+        int g1 = min(sp, g0 + per);
+        int g = g0;
+        for (; g + 4 <= g1; g += 4) {
+            int v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = segs[2 * g + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    alpha, lds + uint32_t(v[2 * j + 1] & 0xffff), 16, lane * 16u, v[2 * j], 0, 0);
+        }
+        for (; g < g1; ++g)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                alpha, lds + uint32_t(segs[2 * g + 1] & 0xffff), 16, lane * 16u, segs[2 * g], 0, 0);
+        per = (s1 - sp + kWaves - 1) / kWaves;
+        g0 = sp + wave * per;
+        g1 = min(s1, g0 + per);
+        for (g = g0; g < g1; g += 4) {
             int v[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) v[i] = segs[2 * g + i];
@@ -167,10 +188,17 @@ struct State {
             }
         }
     }
+    // a stage barrier: this wave's copies into LDS have landed (buffer loads to LDS
+    // count in vmcnt, and the workgroup barrier's fence does not wait for them), then
+    // every wave of the workgroup has arrived
+    __device__ __forceinline__ void stage_barrier() {
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __syncthreads();
+    }
     // every wave is done with the current stage and the next one has landed: move on
     // and start copying the one after into the buffer just released
     __device__ __forceinline__ void advance() {
-        __syncthreads();
+        stage_barrier();
         ++stage;
         next_first = stage_first[stage + 1];
         if (stage + 1 < num_stages) issue(stage + 1);
@@ -513,7 +541,7 @@ void k_eval(Params<T> prm) {
         st.next_first = -1;
         if (st.num_stages > 0) {
             st.issue(0);
-            __syncthreads();
+            st.stage_barrier();
             st.next_first = st.stage_first[1];
             if (st.num_stages > 1) st.issue(1);
         }

@@ -403,6 +403,28 @@ int explicit_tab_dims(const Grid& g) {
 namespace {
 
 // Pack the whole static blocks into LDS stages for one element size (see StagePlan).
+// LDS bytes of a set of leaves laid out in source order, with the leaves that are
+// contiguous in HBM merged into runs and every run starting 16-byte aligned.  leaves
+// holds (base offset, elements) pairs, sorted here.
+int64_t stage_run_bytes(std::vector<std::pair<int32_t, int64_t>>& leaves, int elem) {
+    std::sort(leaves.begin(), leaves.end());
+    int64_t bytes = 0, run = 0;
+    for (size_t i = 0; i < leaves.size(); ++i) {
+        run += leaves[i].second * elem;
+        if (i + 1 == leaves.size() || leaves[i].first + leaves[i].second != leaves[i + 1].first) {
+            bytes += (run + 15) & ~int64_t(15);
+            run = 0;
+        }
+    }
+    return bytes;
+}
+
+// Stages hold whole static blocks, in walk order.  Inside a stage the leaves are laid out
+// in HBM order: in compact order the subspaces below one node of the walk that share a
+// level sum are adjacent, so the leaves of a stage form far fewer contiguous runs than
+// leaves, and each run is copied in segments of at most kStageSegBytes.  A stage lists
+// its full segments (64 lanes) before its partial ones, so the kernel copies most of it
+// without lane masks.
 void build_stage_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 3>>& blocks,
                       int elem, StagePlan& p) {
     p = StagePlan();
@@ -415,36 +437,66 @@ void build_stage_plan(const Grid& g, int J, const std::vector<std::array<int32_t
         size_of[g.sub_base[id]] = end - g.sub_base[id];
     }
     p.leaf_lds.assign(size_t(g.S), -1);
-    int32_t used = kStageBytes;          // no stage open yet
+    // leaves (walk position) of the open stage, and its buffer
+    std::vector<int32_t> open;
     int buf = -1;
+    std::vector<std::pair<int32_t, int64_t>> scratch;
+    std::vector<int32_t> full, part;                     // segment descriptors of a stage
+    auto lay_out = [&]() {
+        if (open.empty()) return;
+        std::vector<std::pair<int32_t, int32_t>> order;     // (base, walk position)
+        order.reserve(open.size());
+        for (int32_t q : open) order.emplace_back(g.dfs_base[q], q);
+        std::sort(order.begin(), order.end());
+        int32_t used = 0;
+        size_t i = 0;
+        while (i < order.size()) {
+            // one run: leaves i..j-1, contiguous in HBM
+            size_t j = i + 1;
+            while (j < order.size() &&
+                   order[j - 1].first + size_of[order[j - 1].first] == order[j].first)
+                ++j;
+            const int32_t run_base = order[i].first;
+            const int32_t run_at = buf * kStageBytes + used;
+            for (size_t k = i; k < j; ++k)
+                p.leaf_lds[order[k].second] = run_at + (order[k].first - run_base) * elem;
+            const int32_t b = int32_t((order[j - 1].first - run_base + size_of[order[j - 1].first]) *
+                                      elem);
+            for (int32_t o = 0; o < b; o += kStageSegBytes) {
+                const int32_t lanes = (std::min(kStageSegBytes, b - o) + 15) / 16;
+                auto& to = lanes == kStageSegBytes / 16 ? full : part;
+                to.push_back(int32_t(int64_t(run_base) * elem + o));
+                to.push_back((run_at + o) | (lanes << 16));
+            }
+            used += (b + 15) & ~15;
+            i = j;
+        }
+        // the stage's full segments, then its partial ones
+        p.seg_begin.push_back(int32_t(p.segs.size() / 2));
+        p.segs.insert(p.segs.end(), full.begin(), full.end());
+        p.seg_begin.push_back(int32_t(p.segs.size() / 2));
+        p.segs.insert(p.segs.end(), part.begin(), part.end());
+        full.clear();
+        part.clear();
+        open.clear();
+    };
     for (const auto& blk : blocks) {
         const int32_t q0 = blk[0], R = blk[1], hsum = blk[2];
         if (!stage_whole(J, R, hsum, elem)) continue;
         const int64_t C = stage_binom(R + J, J);
-        int64_t bytes = 0;
+        scratch.clear();
         for (int64_t c = 0; c < C; ++c)
-            bytes += (size_of[g.dfs_base[q0 + c]] * elem + 15) & ~int64_t(15);
-        if (bytes > kStageBytes) return;                 // the bound above is violated
-        if (used + bytes > kStageBytes) {                // open the next stage
-            buf ^= 1;
-            if (buf < 0) buf = 0;
+            scratch.emplace_back(g.dfs_base[q0 + c], size_of[g.dfs_base[q0 + c]]);
+        if (stage_run_bytes(scratch, elem) > kStageBytes) return;   // the bound is violated
+        for (int32_t q : open) scratch.emplace_back(g.dfs_base[q], size_of[g.dfs_base[q]]);
+        if (open.empty() || stage_run_bytes(scratch, elem) > kStageBytes) {
+            lay_out();                                   // close the stage, open the next
+            buf = buf < 0 ? 0 : buf ^ 1;
             p.stage_first.push_back(q0);
-            p.seg_begin.push_back(int32_t(p.segs.size() / 2));
-            used = 0;
         }
-        for (int64_t c = 0; c < C; ++c) {
-            const int32_t base = g.dfs_base[q0 + c];
-            const int32_t b = int32_t(size_of[base] * elem);
-            const int32_t at = buf * kStageBytes + used;
-            p.leaf_lds[q0 + c] = at;
-            for (int32_t o = 0; o < b; o += kStageSegBytes) {
-                const int32_t piece = std::min(kStageSegBytes, b - o);
-                p.segs.push_back(int32_t(int64_t(base) * elem + o));
-                p.segs.push_back((at + o) | (((piece + 15) / 16) << 16));
-            }
-            used += (b + 15) & ~15;
-        }
+        for (int64_t c = 0; c < C; ++c) open.push_back(int32_t(q0 + c));
     }
+    lay_out();
     p.stage_first.push_back(int32_t(g.S));
     p.seg_begin.push_back(int32_t(p.segs.size() / 2));
     p.segs.insert(p.segs.end(), 8, 0);                   // 4 empty segments of padding

@@ -109,7 +109,8 @@ struct StagePlan {
     bool ok = false;                  // the staged walk may run on this grid
     std::vector<int32_t> leaf_lds;    // [S] LDS byte offset of each staged leaf
     std::vector<int32_t> stage_first; // [K+1] first leaf of stage k; [K] = S
-    std::vector<int32_t> seg_begin;   // [K+1] first copy segment of stage k
+    std::vector<int32_t> seg_begin;   // [2K+1] stage k's full segments (64 lanes) run
+                                      //   from entry 2k to 2k+1, its partial ones to 2k+2
     std::vector<int32_t> segs;        // [nseg + 4][2]: source byte offset,
                                       //   lds offset | lanes << 16 (4 zero entries at the end)
 };

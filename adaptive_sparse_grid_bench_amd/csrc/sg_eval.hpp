@@ -387,12 +387,23 @@ __device__ __forceinline__ void static_block_dispatch(int R, const Ln& ln, T pp,
     }
 }
 
-// The leaves below a depth-(J+1) node whose budget R (at most kUpMax) fits dimension J
+// The leaves below a depth-(J+1) node whose budget R (at most up_max) fits dimension J
 // and the J tabled ones: dimension J's levels 0..R unrolled at compile time, its hat
 // advanced incrementally as in the walk, each level a static block of budget R - l.
-// Saves the walk's per-node dispatch for the many small blocks (FP32 walks: 4-26 % at
-// d = 5..12); leaves, their order and their arithmetic are those of the walk.
-constexpr int kUpMax = 4;
+// Saves the walk's per-node dispatch for the many small blocks; leaves, their order and
+// their arithmetic are those of the walk.
+//   FP32 walks unroll budgets up to 4 (4-26 % faster at d = 5..12).
+//   FP64 registers are tighter: every unrolled budget adds spill traffic, so only the
+//   staged walk unrolls, and less as d grows.  Measured on regular grids, 2^22-2^24
+//   points: budget 3 up to d = 11 (d = 10, n = 8: 182 -> 155 ms), 2 at d = 12, 1 at
+//   d = 13 and 14, none from d = 15 (a budget of 1 there is 6 % slower).  The
+//   gathering FP64 variants do not unroll (up to 2x slower at d = 16).
+template <int D, class T, bool STG>
+constexpr int up_max() {
+    if constexpr (sizeof(T) == 4) return 4;
+    else if constexpr (STG) return D <= 11 ? 3 : D == 12 ? 2 : D <= 14 ? 1 : 0;
+    else return 0;
+}
 
 template <int J, int R, int B, int l, class Ln, class T, class St>
 __device__ __forceinline__ void up_block_level(const Ln& ln, T pp, uint32_t off, int hsum, T m,
@@ -407,12 +418,12 @@ __device__ __forceinline__ void up_block_level(const Ln& ln, T pp, uint32_t off,
     }
 }
 
-template <int J, int LT, int B, class Ln, class T, class St>
+template <int J, int LT, int B, int UP, class Ln, class T, class St>
 __device__ __forceinline__ void up_block_dispatch(int R, const Ln& ln, T m0, T pp, uint32_t off,
                                                   int hsum, St& st) {
     switch (R) {
 #define SG_UP(RR) \
-    case RR: if constexpr (RR < LT && RR <= kUpMax) up_block_level<J, RR, B, 0>(ln, pp, off, hsum, m0, 0u, st); break;
+    case RR: if constexpr (RR < LT && RR <= UP) up_block_level<J, RR, B, 0>(ln, pp, off, hsum, m0, 0u, st); break;
         SG_UP(0) SG_UP(1) SG_UP(2) SG_UP(3) SG_UP(4)
 #undef SG_UP
         default: break;
@@ -445,14 +456,13 @@ struct Walker {
             }
             b = code & 0x7f;
         } else {
-            // FP32 only: with FP64 registers the unrolled levels raise register pressure
-            // enough to cost more than the dispatch they save (measured at d = 5..16)
-            if constexpr (J > 0 && K == J && K < D && sizeof(T) == 4) {
-                bool fits = rem <= kUpMax && rem <= lim[K];
+            constexpr int kUp = up_max<D, T, St::kStaged>();
+            if constexpr (J > 0 && K == J && K < D && kUp > 0) {
+                bool fits = rem <= kUp && rem <= lim[K];
 #pragma unroll
                 for (int j = 0; j < J; ++j) fits = fits && rem <= lim[j];
                 if (fits) {
-                    up_block_dispatch<J, LT, B>(rem, ln, ln.m0[K], pp, off, hsum, st);
+                    up_block_dispatch<J, LT, B, kUp>(rem, ln, ln.m0[K], pp, off, hsum, st);
                     return;
                 }
             }

@@ -394,15 +394,27 @@ __device__ __forceinline__ void static_block_dispatch(int R, const Ln& ln, T pp,
 // their arithmetic are those of the walk.
 //   FP32 walks unroll budgets up to 4 (4-26 % faster at d = 5..12).
 //   FP64 registers are tighter: every unrolled budget adds spill traffic, so only the
-//   staged walk unrolls, and less as d grows.  Measured on regular grids, 2^22-2^24
-//   points: budget 3 up to d = 11 (d = 10, n = 8: 182 -> 155 ms), 2 at d = 12, 1 at
-//   d = 13 and 14, none from d = 15 (a budget of 1 there is 6 % slower).  The
-//   gathering FP64 variants do not unroll (up to 2x slower at d = 16).
+//   staged walk unrolls, and less as d grows (it reads in batches of 4 from d = 9 on,
+//   which frees registers for it).  Measured on regular grids, 2^22-2^24 points: budget
+//   3 up to d = 8, 4 at d = 9 and 10 (d = 10, n = 8: 182 ms unrolled nowhere, 150 ms),
+//   3 at d = 11, 2 at d = 12..14, 1 at d = 15 and 16, none at d = 20 (a budget of 1
+//   there is 16 % slower).  The gathering FP64 variants do not unroll (up to 2x slower
+//   at d = 16).
 template <int D, class T, bool STG>
 constexpr int up_max() {
     if constexpr (sizeof(T) == 4) return 4;
-    else if constexpr (STG) return D <= 11 ? 3 : D == 12 ? 2 : D <= 14 ? 1 : 0;
+    else if constexpr (STG)
+        return D <= 8 ? 3 : D <= 10 ? 4 : D == 11 ? 3 : D <= 14 ? 2 : D <= 16 ? 1 : 0;
     else return 0;
+}
+
+// Batch of the staged walk (leaves whose reads are issued before the first is used).
+// Most of its reads come from LDS and need few in flight; from d = 9 on, FP64 batches of
+// 4 free registers for the unrolled levels (d = 12..16: 9-21 % faster than 8; d = 6..8:
+// 3 % slower).
+template <int D, class T>
+constexpr int stage_batch() {
+    return sizeof(T) == 8 && D >= 9 ? 4 : 8;
 }
 
 template <int J, int R, int B, int l, class Ln, class T, class St>
@@ -568,7 +580,7 @@ struct Variant {
     bool staged;
 };
 constexpr Variant kVariants[] = {{3, 8, 3, false}, {2, 8, 3, false}, {3, 4, 4, false},
-                                 {kStageJ, kStageBatch, 3, true}};
+                                 {kStageJ, 0, 3, true}};   // batch: stage_batch()
 static_assert(kStagedVariant == 3 && kVariants[kStagedVariant].staged, "staged variant index");
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // tabled dimensions the explicit-set kernels use (their node-code stream depends on it)
@@ -594,7 +606,7 @@ hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int64_t blo
     if constexpr (EXPL) {
         static_assert(kExplicitJ == kStageJ, "explicit walk and stage plan table alike");
         if (variant == kStagedVariant)
-            return launch_one<D, 8, kExplicitJ, kStageBatch, 3, true, T, Acc, true>(prm, blocks,
+            return launch_one<D, 8, kExplicitJ, stage_batch<D, T>(), 3, true, T, Acc, true>(prm, blocks,
                                                                                  block, s);
         return launch_one<D, 8, kExplicitJ, 8, 3, true, T, Acc>(prm, blocks, block, s);
     } else {
@@ -602,7 +614,7 @@ hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int64_t blo
             case 1: return launch_one<D, 8, 2, 8, 3, false, T, Acc>(prm, blocks, block, s);
             case 2: return launch_one<D, 8, 3, 4, 4, false, T, Acc>(prm, blocks, block, s);
             case kStagedVariant:
-                return launch_one<D, 8, kStageJ, kStageBatch, 3, false, T, Acc, true>(
+                return launch_one<D, 8, kStageJ, stage_batch<D, T>(), 3, false, T, Acc, true>(
                     prm, blocks, block, s);
             default: return launch_one<D, 8, 3, 8, 3, false, T, Acc>(prm, blocks, block, s);
         }

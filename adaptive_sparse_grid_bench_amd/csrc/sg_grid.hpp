@@ -38,14 +38,13 @@ constexpr int kExplicitTabDims = 3; // register-tabled dims of the explicit-set 
 // Staged evaluation: the workgroup copies surplus blocks into LDS and each lane reads its
 // coefficient there.  Two stage buffers of kStageBytes per workgroup; blocks are copied
 // in segments of at most kStageSegBytes (one wave instruction of 16 bytes per lane).
-// The staged walk tables kStageJ dimensions and reads in batches of kStageBatch leaves.
+// The staged walk tables kStageJ dimensions (its batch size is the kernel's choice).
 // What is staged: the static blocks of the walk (all leaves below a depth-J node whose
 // remaining level budget R fits the tabled dimensions) small enough to share a stage
 // with others, stage_whole(); other leaves are gathered from global memory.
 constexpr int kStageBytes = 24576;
 constexpr int kStageSegBytes = 1024;
 constexpr int kStageJ = 3;
-constexpr int kStageBatch = 8;
 
 SG_HOST_DEVICE constexpr int64_t stage_binom(int a, int b) {
     int64_t r = 1;

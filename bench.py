@@ -28,7 +28,8 @@ METRIC = "eval pts/s & hierarchize GP/s at d=10,n=8; served p50/p99 vs plain FP6
 HBM_PEAK_GBS = 8000.0
 VARIANT_NAMES = {0: "gather, 3 tabled dims, batches of 8", 1: "gather, 2 tabled dims",
                  2: "gather, batches of 4, 4 waves/SIMD",
-                 3: "staged in LDS, 3 tabled dims, batches of 8"}
+                 3: "staged in LDS, 3 tabled dims, FP64 batches of 4 from d=9 (8 below), "
+                    "dimension 3 unrolled below small budgets"}
 
 
 def cpu_baseline(d: int, n: int, P_sample: int, seed: int) -> dict:

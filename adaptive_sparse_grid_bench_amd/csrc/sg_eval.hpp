@@ -417,29 +417,46 @@ constexpr int stage_batch() {
     return sizeof(T) == 8 && D >= 9 ? 4 : 8;
 }
 
-template <int J, int R, int B, int l, class Ln, class T, class St>
+// Up to two dimensions above the tabled ones unroll this way: dimension K's levels
+// 0..R, each level unrolling dimension K-1 with the budget left, down to dimension J
+// whose levels each end in a static block.
+template <int K, int J, int R, int B, int l, class Ln, class T, class St>
 __device__ __forceinline__ void up_block_level(const Ln& ln, T pp, uint32_t off, int hsum, T m,
                                                uint32_t t, St& st) {
     if constexpr (l <= R) {
         const T phi = T(1) - fabs(m);
         const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
-        static_block<J, R - l, B>(ln, pp * phi, o, hsum + l, st);
+        if constexpr (K == J)
+            static_block<J, R - l, B>(ln, pp * phi, o, hsum + l, st);
+        else
+            up_block_level<K - 1, J, R - l, B, 0>(ln, pp * phi, o, hsum + l, ln.m0[K - 1], 0u, st);
         const bool ge = m >= T(0);
-        up_block_level<J, R, B, l + 1>(ln, pp, off, hsum, T(2) * m + (ge ? T(-1) : T(1)),
-                                       2u * t + (ge ? 1u : 0u), st);
+        up_block_level<K, J, R, B, l + 1>(ln, pp, off, hsum, T(2) * m + (ge ? T(-1) : T(1)),
+                                          2u * t + (ge ? 1u : 0u), st);
     }
 }
 
-template <int J, int LT, int B, int UP, class Ln, class T, class St>
-__device__ __forceinline__ void up_block_dispatch(int R, const Ln& ln, T m0, T pp, uint32_t off,
+template <int K, int J, int LT, int B, int UP, class Ln, class T, class St>
+__device__ __forceinline__ void up_block_dispatch(int R, const Ln& ln, T pp, uint32_t off,
                                                   int hsum, St& st) {
     switch (R) {
 #define SG_UP(RR) \
-    case RR: if constexpr (RR < LT && RR <= UP) up_block_level<J, RR, B, 0>(ln, pp, off, hsum, m0, 0u, st); break;
+    case RR: if constexpr (RR < LT && RR <= UP) up_block_level<K, J, RR, B, 0>(ln, pp, off, hsum, ln.m0[K], 0u, st); break;
         SG_UP(0) SG_UP(1) SG_UP(2) SG_UP(3) SG_UP(4)
 #undef SG_UP
         default: break;
     }
+}
+
+// Largest budget for which dimensions J+1 and J both unroll (0: never).  It pays only
+// for the FP64 staged walk at d = 10..13 (ms, regular grids, budget 0 / 1 / 2: d = 10,
+// n = 8, 2^24 points 150.7 / 153.2 / 148.7; d = 11, n = 7, 2^22 points 26.7 / 25.7 /
+// 26.7; d = 12, n = 7: 45.6 / 44.9 / 43.7; d = 13, n = 6: 28.0 / 27.5 / 30.5); elsewhere
+// the extra live registers cost up to 1.6x (d = 20).
+template <int D, class T, bool STG>
+constexpr int up2_max() {
+    if constexpr (sizeof(T) == 8 && STG) return D == 10 || D == 12 ? 2 : D == 11 || D == 13 ? 1 : 0;
+    else return 0;
 }
 
 // Depth-first walk.  Dimension K's level runs 0..bound, where bound = min(rem,
@@ -468,13 +485,23 @@ struct Walker {
             }
             b = code & 0x7f;
         } else {
+            constexpr int kUp2 = up2_max<D, T, St::kStaged>();
+            if constexpr (J > 0 && K == J + 1 && K < D && kUp2 > 0) {
+                bool fits = rem <= kUp2 && rem <= lim[K] && rem <= lim[K - 1];
+#pragma unroll
+                for (int j = 0; j < J; ++j) fits = fits && rem <= lim[j];
+                if (fits) {
+                    up_block_dispatch<K, J, LT, B, kUp2>(rem, ln, pp, off, hsum, st);
+                    return;
+                }
+            }
             constexpr int kUp = up_max<D, T, St::kStaged>();
             if constexpr (J > 0 && K == J && K < D && kUp > 0) {
                 bool fits = rem <= kUp && rem <= lim[K];
 #pragma unroll
                 for (int j = 0; j < J; ++j) fits = fits && rem <= lim[j];
                 if (fits) {
-                    up_block_dispatch<J, LT, B, kUp>(rem, ln, ln.m0[K], pp, off, hsum, st);
+                    up_block_dispatch<K, J, LT, B, kUp>(rem, ln, pp, off, hsum, st);
                     return;
                 }
             }

@@ -448,15 +448,18 @@ __device__ __forceinline__ void up_block_dispatch(int R, const Ln& ln, T pp, uin
     }
 }
 
-// Largest budget for which dimensions J+1 and J both unroll (0: never).  It pays only
-// for the FP64 staged walk at d = 10..13 (ms, regular grids, budget 0 / 1 / 2: d = 10,
-// n = 8, 2^24 points 150.7 / 153.2 / 148.7; d = 11, n = 7, 2^22 points 26.7 / 25.7 /
-// 26.7; d = 12, n = 7: 45.6 / 44.9 / 43.7; d = 13, n = 6: 28.0 / 27.5 / 30.5); elsewhere
-// the extra live registers cost up to 1.6x (d = 20).
+// Largest budget for which dimensions J+1 and J both unroll (0: never); staged walks
+// only.  Measured on regular grids (ms, 2^24 points at d = 10, 2^22 otherwise):
+//   FP64, budget 0 / 1 / 2: d = 10, n = 8: 150.7 / 153.2 / 148.7; d = 11, n = 7: 26.7 /
+//   25.7 / 26.7; d = 12, n = 7: 45.6 / 44.9 / 43.7; d = 13, n = 6: 28.0 / 27.5 / 30.5;
+//   elsewhere the extra live registers cost up to 1.6x (d = 20).
+//   FP32 (Kahan), budget 0 / 2 / 4: d = 10: 124.1 / 111.0 / 108.5; d = 12: 38.8 / 32.4 /
+//   31.5; d = 14, n = 6: 36.7 / 28.9 / 28.7; d = 16, n = 5: 26.1 / 20.2 / 20.5.
 template <int D, class T, bool STG>
 constexpr int up2_max() {
-    if constexpr (sizeof(T) == 8 && STG) return D == 10 || D == 12 ? 2 : D == 11 || D == 13 ? 1 : 0;
-    else return 0;
+    if constexpr (!STG) return 0;
+    else if constexpr (sizeof(T) == 4) return 4;
+    else return D == 10 || D == 12 ? 2 : D == 11 || D == 13 ? 1 : 0;
 }
 
 // Depth-first walk.  Dimension K's level runs 0..bound, where bound = min(rem,

@@ -54,7 +54,12 @@ def run_variants(torch, g, alpha, pts, dtype, binning):
 
 SHAPES = [(10, 8, None, 3000), (10, 8, None, 65), (4, 10, None, 2000), (1, 8, None, 700),
           (2, 8, None, 1), (1, 1, None, 300), (20, 4, None, 1500), (3, 5, None, 257),
-          (6, 7, [3, 1, 3, 2, 6, 6], 2500), (3, 6, [1, 5, 5], 999), (3, 11, None, 600)]
+          (6, 7, [3, 1, 3, 2, 6, 6], 2500), (3, 6, [1, 5, 5], 999), (3, 11, None, 600),
+          # every dimension-dependent form of the walk: batches of 8 (d <= 8) or 4, the
+          # unroll caps of dimension 3 (d = 9..16) and of dimensions 4 and 3 (d = 10..13)
+          (8, 6, None, 700), (9, 6, None, 700), (11, 5, None, 600), (12, 5, None, 600),
+          (13, 5, None, 500), (14, 4, None, 500), (15, 4, None, 500), (16, 4, None, 500),
+          (12, 6, [5, 5, 5, 5, 2, 5, 5, 5, 5, 5, 5, 5], 400)]
 
 
 @pytest.mark.parametrize("d,n,lim,P", SHAPES,

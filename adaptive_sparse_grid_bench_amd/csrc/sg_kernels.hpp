@@ -117,7 +117,7 @@ constexpr int kStagedVariant = 3;
 // default variants by dimension and precision, from timings at 2^22-2^24 points on
 // d = 5..16 and 20 (the autotuner refines them per grid shape)
 inline int default_variant(int d, bool f32) {
-    if (f32) return d <= 16 ? kStagedVariant : 2;
+    if (f32) return kStagedVariant;
     return d >= 17 ? 1 : kStagedVariant;
 }
 // the variant a launch runs: the staged walk falls back to variant 0 for a grid whose

@@ -5,9 +5,10 @@ Bars: the staged walk visits the same leaves in the same order and does the same
 arithmetic per point as variant 0 (three tabled dimensions, batches of 8), so its
 outputs are bit-identical to variant 0's in FP64 and FP32-Kahan; and they satisfy the
 FP64 bound against the oracle, |gpu - oracle| <= 1e-12 * sum_j |term_j|.  Shapes cover
-one and many stages, blocks of exactly 4 KiB (the largest staged block), blocks cut by
-restrictions (single-leaf batches), grids that cannot be staged (variant 0 runs),
-explicit level sets, and workgroups with idle lanes."""
+one and many stages, blocks too large for a stage (gathered inside the staged walk),
+blocks cut by restrictions (single-leaf batches), grids that cannot be staged (variant 0
+runs), explicit level sets, workgroups with idle lanes, and every dimension-dependent
+form of the walk."""
 import numpy as np
 import pytest
 

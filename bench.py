@@ -306,6 +306,18 @@ def main() -> int:
         },
         "hierarchize_gp_per_s": g.N / (hier_med * 1e-3),
         "hierarchize_ms": hier_med,
+        # algorithmic bytes of a full hierarchization: every value read and written once
+        # per dimension sweep (2 d N 8 bytes); the pair kernel moves each value once per
+        # two dimensions, so its HBM bytes are about half of these
+        "hierarchize_roofline": {
+            "kernels": "k_hierarchize_pair (two dimensions per launch), k_hierarchize_dim",
+            "bound": "hbm",
+            "algorithmic_bytes": 2 * d * g.N * 8,
+            "achieved": 2 * d * g.N * 8 / (hier_med * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": 2 * d * g.N * 8 / (hier_med * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
         "evaluate_ms": eval_avg,
         "evaluate_pts_per_s_kernel": P / (eval_avg * 1e-3),
         "roofline": {

@@ -102,7 +102,7 @@ int Batch::build(std::vector<GridRef> members) {
     };
     auto add_pair_sweep = [&](int k) -> int {
         std::vector<int32_t> tiles, fams, subs, blocks;
-        int tv = 0, mf = 0, ms = 0;
+        int tv = 0, mf = 0, ms = 0, mb = 0;
         for (int i = 0; i < G; ++i) {
             const Grid& g = *grids[i].g;
             if (!uses_pair(g, k)) continue;
@@ -127,6 +127,7 @@ int Batch::build(std::vector<GridRef> members) {
             for (int32_t b : h.blocks) blocks.push_back(int32_t(b + value_off[i]));
             mf = std::max(mf, h.max_families_per_tile);
             ms = std::max(ms, h.max_subs_per_tile);
+            mb = std::max(mb, h.max_blocks_per_tile);
         }
         if (tiles.empty()) return kOk;
         HierDim hd;
@@ -135,6 +136,7 @@ int Batch::build(std::vector<GridRef> members) {
         hd.tile_values = tv;
         hd.max_families = mf;
         hd.max_subs = ms;
+        hd.max_blocks = mb;
         int st;
         if ((st = upload_vec(hd.tiles, tiles)) != kOk) return st;
         if ((st = upload_vec(hd.fams, fams)) != kOk) return st;
@@ -210,7 +212,8 @@ int Batch::hierarchize(T* v, hipStream_t s) const {
         if (hd.num_tiles == 0) continue;
         int st = hd.pair
             ? launch_hierarchize_pairs<T>(v, hd.tiles, hd.fams, hd.subs, hd.blocks, hd.num_tiles,
-                                          hd.tile_values, hd.max_families, hd.max_subs, s)
+                                          hd.tile_values, hd.max_families, hd.max_subs,
+                                          hd.max_blocks, s)
             : launch_hierarchize_tiles<T>(v, hd.tiles, hd.fams, hd.blocks, hd.num_tiles,
                                           hd.tile_values, hd.max_families, s);
         if (st != kOk) return st;

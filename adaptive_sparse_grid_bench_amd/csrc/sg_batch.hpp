@@ -31,6 +31,7 @@ struct Batch {
         int tile_values = 0;
         int max_families = 0;
         int max_subs = 0;
+        int max_blocks = 0;
     };
     struct Group {               // one evaluation launch
         bool explicit_set;

@@ -593,7 +593,7 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
         const int k = 2 * p;
         PairTables& h = g.hier2[p];
         const int tile_target = std::min(std::max(hier_tile_values, 64), kPairMaxValues);
-        int32_t tile_first = 0, tile_fams = 0, tile_vals = 0, tile_subs = 0;
+        int32_t tile_first = 0, tile_fams = 0, tile_vals = 0, tile_subs = 0, tile_blk0 = 0;
         auto close_tile = [&]() {
             if (tile_fams == 0) return;
             h.tiles.push_back(tile_first);
@@ -601,6 +601,8 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
             h.tiles.push_back(tile_vals);
             h.max_families_per_tile = std::max(h.max_families_per_tile, int(tile_fams));
             h.max_subs_per_tile = std::max(h.max_subs_per_tile, int(tile_subs));
+            h.max_blocks_per_tile =
+                std::max(h.max_blocks_per_tile, int(h.blocks.size()) - int(tile_blk0));
             h.max_tile_values = std::max(h.max_tile_values, int(tile_vals));
             tile_first += tile_fams;
             tile_fams = 0;
@@ -646,6 +648,7 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
             }
             if (nblocks == 1) continue;                    // one subspace: no parents
             if (tile_fams > 0 && tile_vals + vals > tile_target) close_tile();
+            if (tile_fams == 0) tile_blk0 = int32_t(h.blocks.size());
             h.fams.push_back(tile_vals);
             h.fams.push_back(c | (Bfix << 8) | ((int(sub_first.size()) - 1) << 16));
             h.fams.push_back(int32_t(h.subs.size() / 2));
@@ -659,6 +662,8 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
             ++tile_fams;
         }
         close_tile();
+        h.subs.push_back(0);                               // sentinel: the blocks' end
+        h.subs.push_back(int32_t(h.blocks.size()));
         h.ok = fits;
         if (!fits) h = PairTables();
     }

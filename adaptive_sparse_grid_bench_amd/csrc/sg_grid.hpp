@@ -93,10 +93,12 @@ struct PairTables {
     std::vector<int32_t> tiles;       // [T][3]: first family, family count, value count
     std::vector<int32_t> fams;        // [F][3]: first value in the tile, c|Bfix<<8|A<<16,
                                       //          first sub-pole
-    std::vector<int32_t> subs;        // [P][2]: first value in the family, first block
+    std::vector<int32_t> subs;        // [P+1][2]: first value in the family, first block;
+                                      //   entry P (a sentinel) holds the blocks' end
     std::vector<int32_t> blocks;      // base offset of each block
     int max_families_per_tile = 0;
     int max_subs_per_tile = 0;
+    int max_blocks_per_tile = 0;
     int max_tile_values = 0;
 };
 

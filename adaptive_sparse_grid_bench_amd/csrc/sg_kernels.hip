@@ -361,13 +361,15 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v
                                                                const int32_t* __restrict__ blocks,
                                                                int lds_values, int lds_fams,
                                                                int lds_subs) {
+    // the tile's block bases follow the sub-pole tables in LDS
     extern __shared__ __align__(16) unsigned char smem[];
     T* buf = reinterpret_cast<T*>(smem);
     int32_t* fst = reinterpret_cast<int32_t*>(smem + size_t(lds_values) * sizeof(T));
     int32_t* fmeta = fst + lds_fams;
     int32_t* fsub = fmeta + lds_fams;
     int32_t* sfirst = fsub + lds_fams;          // sub-pole starts within their family
-    int32_t* sblock = sfirst + lds_subs;        // sub-pole first block
+    int32_t* sblock = sfirst + lds_subs;        // sub-pole first block, within the tile
+    int32_t* sblk = sblock + lds_subs;          // the tile's block bases
     const int f0 = tiles[3 * blockIdx.x];
     const int nf = tiles[3 * blockIdx.x + 1];
     const int nv = tiles[3 * blockIdx.x + 2];
@@ -380,10 +382,15 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v
     // the tile's sub-poles: from the first family's to the last family's last one
     const int last_meta = fams[3 * (f0 + nf - 1) + 1];
     const int nsub = fams[3 * (f0 + nf - 1) + 2] - s0 + (last_meta >> 16) + 1;
+    // the tile's blocks are consecutive: from its first sub-pole's first block to the
+    // next sub-pole's (the table ends in a sentinel sub-pole)
+    const int b0 = subs[2 * s0 + 1];
+    const int nb = subs[2 * (s0 + nsub) + 1] - b0;
     for (int i = threadIdx.x; i < nsub; i += blockDim.x) {
         sfirst[i] = subs[2 * (s0 + i)];
-        sblock[i] = subs[2 * (s0 + i) + 1];
+        sblock[i] = subs[2 * (s0 + i) + 1] - b0;
     }
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) sblk[i] = blocks[b0 + i];
     __syncthreads();
 
     int pf[PER], pa[PER], pb[PER], pr[PER];
@@ -411,7 +418,7 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v
             pa[i] = a;
             pb[i] = b;
             pr[i] = sl - (((1 << b) - 1) << cc);
-            buf[val] = v[blocks[sblock[sf + a] + b] + pr[i]];
+            buf[val] = v[sblk[sblock[sf + a] + b] + pr[i]];
         }
     }
     __syncthreads();
@@ -493,7 +500,7 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v
                 }
                 x = x - (left + right) * T(0.5);
             }
-            v[blocks[sblock[sf + a] + b] + r] = x;
+            v[sblk[sblock[sf + a] + b] + r] = x;
         }
     }
 }
@@ -503,14 +510,14 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v
 template <class T>
 int launch_hierarchize_pairs(T* v, const int32_t* tiles, const int32_t* fams, const int32_t* subs,
                              const int32_t* blocks, int num_tiles, int tile_values,
-                             int max_families, int max_subs, hipStream_t s) {
+                             int max_families, int max_subs, int max_blocks, hipStream_t s) {
     if (tile_values > kPairMaxValues) {
         set_error("hierarchize: plane-family tile larger than the kernel's limit");
         return kUnsupported;
     }
-    // LDS: the largest tile's values, family and sub-pole tables
+    // LDS: the largest tile's values, family, sub-pole and block tables
     const size_t lds = size_t(tile_values) * sizeof(T) + size_t(max_families) * 12 +
-                       size_t(max_subs) * 8;
+                       size_t(max_subs) * 8 + size_t(max_blocks) * 4;
     const int per = (tile_values + kThreads - 1) / kThreads;
 #define SG_PAIR(PER)                                                                        \
     hipLaunchKernelGGL((k_hierarchize_pair<T, PER>), dim3(num_tiles), dim3(kThreads), lds, s, \
@@ -524,10 +531,10 @@ int launch_hierarchize_pairs(T* v, const int32_t* tiles, const int32_t* fams, co
 }
 template int launch_hierarchize_pairs<double>(double*, const int32_t*, const int32_t*,
                                               const int32_t*, const int32_t*, int, int, int, int,
-                                              hipStream_t);
+                                              int, hipStream_t);
 template int launch_hierarchize_pairs<float>(float*, const int32_t*, const int32_t*,
                                              const int32_t*, const int32_t*, int, int, int, int,
-                                             hipStream_t);
+                                             int, hipStream_t);
 
 namespace {
 
@@ -540,7 +547,8 @@ int hierarchize_impl(const Grid& g, const DeviceTables& t, T* v, hipStream_t s) 
             if (pr.num_tiles > 0) {
                 int st = launch_hierarchize_pairs<T>(v, pr.tiles, pr.fams, pr.subs, pr.blocks,
                                                      pr.num_tiles, pr.tile_values,
-                                                     pr.max_families, pr.max_subs, s);
+                                                     pr.max_families, pr.max_subs,
+                                                     pr.max_blocks, s);
                 if (st != kOk) return st;
             }
             k += 2;
@@ -904,6 +912,7 @@ int upload_tables(const Grid& g, DeviceTables& t) {
         pr.num_tiles = int(h.tiles.size() / 3);
         pr.max_families = h.max_families_per_tile;
         pr.max_subs = h.max_subs_per_tile;
+        pr.max_blocks = h.max_blocks_per_tile;
         pr.tile_values = h.max_tile_values;
         if ((st = upload(pr.tiles, h.tiles.data(), h.tiles.size())) != kOk) return st;
         if ((st = upload(pr.fams, h.fams.data(), h.fams.size())) != kOk) return st;

@@ -78,6 +78,7 @@ struct DeviceTables {
         int num_tiles = 0;
         int max_families = 0;
         int max_subs = 0;
+        int max_blocks = 0;
         int tile_values = 0;
     } pairs[kMaxDims / 2];
     int hier_tile_values = 0;
@@ -103,7 +104,7 @@ int launch_hierarchize_f32(const Grid& g, const DeviceTables& t, float* v, hipSt
 template <class T>
 int launch_hierarchize_pairs(T* v, const int32_t* tiles, const int32_t* fams, const int32_t* subs,
                              const int32_t* blocks, int num_tiles, int tile_values,
-                             int max_families, int max_subs, hipStream_t s);
+                             int max_families, int max_subs, int max_blocks, hipStream_t s);
 
 struct EvalTuning {
     int block = 256;            // threads per workgroup

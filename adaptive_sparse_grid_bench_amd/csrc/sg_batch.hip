@@ -109,10 +109,15 @@ int Batch::build(std::vector<GridRef> members) {
             const PairTables& h = g.hier2[k / 2];
             const int32_t fam0 = int32_t(fams.size() / 3), sub0 = int32_t(subs.size() / 2),
                           blk0 = int32_t(blocks.size());
-            for (size_t t = 0; t < h.tiles.size(); t += 3) {
+            for (size_t t = 0; t < h.tiles.size(); t += kPairTileInts) {
                 tiles.push_back(h.tiles[t] + fam0);
                 tiles.push_back(h.tiles[t + 1]);
                 tiles.push_back(h.tiles[t + 2]);
+                tiles.push_back(h.tiles[t + 3] + sub0);
+                tiles.push_back(h.tiles[t + 4]);
+                tiles.push_back(h.tiles[t + 5] + blk0);
+                tiles.push_back(h.tiles[t + 6]);
+                tiles.push_back(0);
                 tv = std::max(tv, int(h.tiles[t + 2]));
             }
             for (size_t f = 0; f < h.fams.size(); f += 3) {
@@ -132,7 +137,7 @@ int Batch::build(std::vector<GridRef> members) {
         if (tiles.empty()) return kOk;
         HierDim hd;
         hd.pair = true;
-        hd.num_tiles = int(tiles.size() / 3);
+        hd.num_tiles = int(tiles.size() / kPairTileInts);
         hd.tile_values = tv;
         hd.max_families = mf;
         hd.max_subs = ms;

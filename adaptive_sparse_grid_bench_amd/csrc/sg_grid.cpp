@@ -599,6 +599,11 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
             h.tiles.push_back(tile_first);
             h.tiles.push_back(tile_fams);
             h.tiles.push_back(tile_vals);
+            h.tiles.push_back(h.fams[3 * tile_first + 2]);
+            h.tiles.push_back(tile_subs);
+            h.tiles.push_back(tile_blk0);
+            h.tiles.push_back(int32_t(h.blocks.size()) - tile_blk0);
+            h.tiles.push_back(0);
             h.max_families_per_tile = std::max(h.max_families_per_tile, int(tile_fams));
             h.max_subs_per_tile = std::max(h.max_subs_per_tile, int(tile_subs));
             h.max_blocks_per_tile =
@@ -662,8 +667,6 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
             ++tile_fams;
         }
         close_tile();
-        h.subs.push_back(0);                               // sentinel: the blocks' end
-        h.subs.push_back(int32_t(h.blocks.size()));
         h.ok = fits;
         if (!fits) h = PairTables();
     }

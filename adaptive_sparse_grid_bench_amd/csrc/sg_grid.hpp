@@ -88,13 +88,16 @@ struct HierTables {
 // dimensions' parents of every point lie in its family, so one workgroup sweeps k and
 // then k+1 in LDS.
 constexpr int kPairMaxValues = 4096;  // largest tile (and family) of the pair sweep
+constexpr int kPairTileInts = 8;       // ints per pair tile entry
 struct PairTables {
     bool ok = false;                  // every family fits kPairMaxValues
-    std::vector<int32_t> tiles;       // [T][3]: first family, family count, value count
+    std::vector<int32_t> tiles;       // [T][8]: first family, family count, value count,
+                                      //   first sub-pole, sub-pole count, first block,
+                                      //   block count, 0 (a tile's sub-poles and blocks
+                                      //   are consecutive)
     std::vector<int32_t> fams;        // [F][3]: first value in the tile, c|Bfix<<8|A<<16,
                                       //          first sub-pole
-    std::vector<int32_t> subs;        // [P+1][2]: first value in the family, first block;
-                                      //   entry P (a sentinel) holds the blocks' end
+    std::vector<int32_t> subs;        // [P][2]: first value in the family, first block
     std::vector<int32_t> blocks;      // base offset of each block
     int max_families_per_tile = 0;
     int max_subs_per_tile = 0;

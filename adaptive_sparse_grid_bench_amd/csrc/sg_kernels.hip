@@ -361,7 +361,6 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v
                                                                const int32_t* __restrict__ blocks,
                                                                int lds_values, int lds_fams,
                                                                int lds_subs) {
-    // the tile's block bases follow the sub-pole tables in LDS
     extern __shared__ __align__(16) unsigned char smem[];
     T* buf = reinterpret_cast<T*>(smem);
     int32_t* fst = reinterpret_cast<int32_t*>(smem + size_t(lds_values) * sizeof(T));
@@ -370,22 +369,16 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_pair(T* __restrict__ v
     int32_t* sfirst = fsub + lds_fams;          // sub-pole starts within their family
     int32_t* sblock = sfirst + lds_subs;        // sub-pole first block, within the tile
     int32_t* sblk = sblock + lds_subs;          // the tile's block bases
-    const int f0 = tiles[3 * blockIdx.x];
-    const int nf = tiles[3 * blockIdx.x + 1];
-    const int nv = tiles[3 * blockIdx.x + 2];
-    const int s0 = fams[3 * f0 + 2];
+    // the tile's entry names its families, sub-poles and blocks (each consecutive), so
+    // the three tables load side by side
+    const int32_t* te = tiles + kPairTileInts * blockIdx.x;
+    const int f0 = te[0], nf = te[1], nv = te[2];
+    const int s0 = te[3], nsub = te[4], b0 = te[5], nb = te[6];
     for (int i = threadIdx.x; i < nf; i += blockDim.x) {
         fst[i] = fams[3 * (f0 + i)];
         fmeta[i] = fams[3 * (f0 + i) + 1];
         fsub[i] = fams[3 * (f0 + i) + 2] - s0;
     }
-    // the tile's sub-poles: from the first family's to the last family's last one
-    const int last_meta = fams[3 * (f0 + nf - 1) + 1];
-    const int nsub = fams[3 * (f0 + nf - 1) + 2] - s0 + (last_meta >> 16) + 1;
-    // the tile's blocks are consecutive: from its first sub-pole's first block to the
-    // next sub-pole's (the table ends in a sentinel sub-pole)
-    const int b0 = subs[2 * s0 + 1];
-    const int nb = subs[2 * (s0 + nsub) + 1] - b0;
     for (int i = threadIdx.x; i < nsub; i += blockDim.x) {
         sfirst[i] = subs[2 * (s0 + i)];
         sblock[i] = subs[2 * (s0 + i) + 1] - b0;
@@ -909,7 +902,7 @@ int upload_tables(const Grid& g, DeviceTables& t) {
         DeviceTables::Pair& pr = t.pairs[p];
         pr.ok = h.ok;
         if (!h.ok) continue;
-        pr.num_tiles = int(h.tiles.size() / 3);
+        pr.num_tiles = int(h.tiles.size() / kPairTileInts);
         pr.max_families = h.max_families_per_tile;
         pr.max_subs = h.max_subs_per_tile;
         pr.max_blocks = h.max_blocks_per_tile;

@@ -396,15 +396,15 @@ __device__ __forceinline__ void static_block_dispatch(int R, const Ln& ln, T pp,
 //   FP64 registers are tighter: every unrolled budget adds spill traffic, so only the
 //   staged walk unrolls, and less as d grows (it reads in batches of 4 from d = 9 on,
 //   which frees registers for it).  Measured on regular grids, 2^22-2^24 points: budget
-//   3 up to d = 8, 4 at d = 9 and 10 (d = 10, n = 8: 182 ms unrolled nowhere, 150 ms),
-//   3 at d = 11, 2 at d = 12..14, 1 at d = 15 and 16, none at d = 20 (a budget of 1
-//   there is 16 % slower).  The gathering FP64 variants do not unroll (up to 2x slower
-//   at d = 16).
+//   3 up to d = 8, 4 at d = 9, 3 at d = 10 and 11 (d = 10, n = 8: 182 ms unrolled
+//   nowhere, 147 ms; with budget 4 there, 149 ms), 2 at d = 12..14, 1 at d = 15 and 16,
+//   none at d = 20 (a budget of 1 there is 16 % slower).  The gathering FP64 variants
+//   do not unroll (up to 2x slower at d = 16).
 template <int D, class T, bool STG>
 constexpr int up_max() {
     if constexpr (sizeof(T) == 4) return 4;
     else if constexpr (STG)
-        return D <= 8 ? 3 : D <= 10 ? 4 : D == 11 ? 3 : D <= 14 ? 2 : D <= 16 ? 1 : 0;
+        return D <= 8 ? 3 : D == 9 ? 4 : D <= 11 ? 3 : D <= 14 ? 2 : D <= 16 ? 1 : 0;
     else return 0;
 }
 

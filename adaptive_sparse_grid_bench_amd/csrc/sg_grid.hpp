@@ -42,7 +42,7 @@ constexpr int kExplicitTabDims = 3; // register-tabled dims of the explicit-set 
 // What is staged: the static blocks of the walk (all leaves below a depth-J node whose
 // remaining level budget R fits the tabled dimensions) small enough to share a stage
 // with others, stage_whole(); other leaves are gathered from global memory.
-constexpr int kStageBytes = 24576;
+constexpr int kStageBytes = 26624;   // 3 workgroups x 2 stages fill 156 KiB of a CU's 160
 constexpr int kStageSegBytes = 1024;
 constexpr int kStageJ = 3;
 

@@ -85,7 +85,7 @@ int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *bas
  *                     coefficient per lane from HBM/cache (they differ in tabled
  *                     dimensions, gathers per batch and waves per SIMD); 3 is the staged
  *                     walk, where the workgroup copies surplus blocks into LDS and the
- *                     lanes read them there (blocks too large for a 24 KiB stage are
+ *                     lanes read them there (blocks too large for a 26 KiB stage are
  *                     gathered as in 0..2).  A grid whose three tabled dimensions reach
  *                     level 8 runs variant 0 instead of 3;
  *   bin_min_points    evaluate launches with at least this many points bin the points

@@ -2,8 +2,8 @@
 
 For a grid it times, on the GPU and with HIP events on the launching stream:
   * the evaluate kernel over the compiled walk variants (tabled dimensions, gathers per
-    batch, waves per SIMD), the workgroup size and spatial binning on or off, in FP64
-    and FP32 (Kahan);
+    batch, waves per SIMD, the staged and block-staged walks), the workgroup size and
+    spatial binning on or off, in FP64 and FP32 (Kahan);
   * the hierarchization over the pole-family tile size (values per workgroup).
 The fastest choice is applied with ``Grid.set_tuning`` and stored in a JSON cache keyed
 by the grid shape (d, n, restriction vector or a digest of the explicit level set), so a
@@ -95,10 +95,10 @@ def tune(g: "sgrid.Grid", points: int = 1 << 17, reps: int = 3, precisions=("f64
         key = "eval_variant_f64" if prec == "f64" else "eval_variant_f32"
         cand = []
         for variant in range(sgrid.EVAL_VARIANTS):
-            # the staged walk always runs 256-thread workgroups
-            for block in ((256,) if variant == sgrid.STAGED_VARIANT else EVAL_BLOCKS):
-                # the staged walk never bins
-                for binning in ((-1,) if variant == sgrid.STAGED_VARIANT else (-1, 4096)):
+            staged = variant in (sgrid.STAGED_VARIANT, sgrid.BLK_VARIANT)
+            # the staged walks choose their workgroup size themselves and never bin
+            for block in ((256,) if staged else EVAL_BLOCKS):
+                for binning in ((-1,) if staged else (-1, 4096)):
                     g.set_tuning(**{**base, key: variant, "eval_block": block,
                                     "bin_min_points": binning})
                     ms = _time(torch, lambda: g.evaluate(a, x, o), reps)

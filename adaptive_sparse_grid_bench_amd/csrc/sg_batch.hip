@@ -57,6 +57,16 @@ int Batch::build(std::vector<GridRef> members) {
         set_error("batch: more than 2^31 grid points in total");
         return kRange;
     }
+    // the evaluate kernels address one grid's surpluses through a buffer of at most
+    // 2 GiB (as a single-grid evaluate does)
+    max_grid_values = 0;
+    for (int i = 0; i < G; ++i) {
+        if (grids[i].g->N * int64_t(sizeof(float)) > int64_t(0x7fffffff)) {
+            set_error("batch: grid " + std::to_string(i) + "'s surpluses exceed 2 GiB");
+            return kRange;
+        }
+        max_grid_values = std::max(max_grid_values, grids[i].g->N);
+    }
     int maxd = 0;
     for (auto& r : grids) maxd = std::max(maxd, r.g->d);
     // Sweeps in launch order.  For each dimension pair (k, k+1): one plane-family sweep
@@ -185,8 +195,8 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
         // table length valid for every variant (variant 0 tables the most dimensions)
         const int lt = eval::table_levels(g, 0);
         auto& items = groups_h[std::make_tuple(g.explicit_set ? 1 : 0, g.d, lt,
-                                               effective_variant(g, grids[i].variant_f64, 8),
-                                               effective_variant(g, grids[i].variant_f32, 4))];
+                                               work_list_variant(g, grids[i].variant_f64, 8),
+                                               work_list_variant(g, grids[i].variant_f32, 4))];
         for (int64_t f = 0; f < counts[i]; f += eval::kThreads)
             items.push_back({i, int32_t(f), int32_t(std::min<int64_t>(eval::kThreads, counts[i] - f)), 0});
     }
@@ -239,7 +249,8 @@ int Batch::evaluate_group(const Group& gr, const T* alpha, const T* coords, T* o
     prm.P = 0;
     const int variant = sizeof(T) == 4 ? gr.variant_f32 : gr.variant_f64;
     hipError_t e = gr.explicit_set
-        ? eval::launch_d<true, T, Acc>(gr.d, 0, gr.lt, prm, gr.count, eval::kThreads, s)
+        ? eval::launch_d<true, T, Acc>(gr.d, variant == kStagedVariant ? variant : 0, gr.lt,
+                                       prm, gr.count, eval::kThreads, s)
         : eval::launch_d<false, T, Acc>(gr.d, variant, gr.lt, prm, gr.count, eval::kThreads, s);
     if (e != hipSuccess) {
         set_error(std::string("batch evaluate launch: ") + hipGetErrorString(e));
@@ -248,8 +259,18 @@ int Batch::evaluate_group(const Group& gr, const T* alpha, const T* coords, T* o
     return kOk;
 }
 
+template <class T>
+int Batch::check_range() const {
+    if (max_grid_values * int64_t(sizeof(T)) > int64_t(0x7fffffff)) {
+        set_error("batch evaluate: a grid's surpluses exceed 2 GiB in this precision");
+        return kRange;
+    }
+    return kOk;
+}
+
 template <class T, class Acc>
 int Batch::evaluate(const T* alpha, const T* coords, T* out, hipStream_t s) const {
+    if (int st = check_range<T>(); st != kOk) return st;
     for (const Group& gr : groups) {
         int st = evaluate_group<T, Acc>(gr, alpha, coords, out, s);
         if (st != kOk) return st;
@@ -262,6 +283,7 @@ int Batch::evaluate(const T* alpha, const T* coords, T* out, hipStream_t s) cons
 template <class T, class Acc>
 int Batch::run_graph(T* alpha, const T* coords, T* out, bool with_hierarchize,
                      hipStream_t s) {
+    if (int st = check_range<T>(); st != kOk) return st;
     std::lock_guard<std::mutex> lock(mu);
     const GraphKey key{static_cast<const void*>(alpha), static_cast<const void*>(coords),
                        static_cast<const void*>(out), with_hierarchize, int(sizeof(T))};

@@ -46,6 +46,7 @@ struct Batch {
 
     std::vector<GridRef> grids;
     std::vector<int64_t> value_off;   // [G+1]: grid g's surpluses start here
+    int64_t max_grid_values = 0;      // largest member grid
     std::vector<int64_t> point_off;   // [G+1]: grid g's points / outputs start here
     std::vector<int64_t> coord_off;   // [G+1]: grid g's coordinates start here (elements)
     std::vector<HierDim> hier;        // the sweeps, in launch order
@@ -70,6 +71,8 @@ struct Batch {
 
     template <class T>
     int hierarchize(T* v, hipStream_t s) const;
+    template <class T>
+    int check_range() const;
     template <class T, class Acc>
     int evaluate(const T* alpha, const T* coords, T* out, hipStream_t s) const;
     template <class T, class Acc>

@@ -95,7 +95,7 @@ int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, v
     const int v = sg::effective_variant(h->g, sizeof(T) == 8 ? h->tune.variant : h->tune32.variant,
                                         int(sizeof(T)));
     if (P < h->tune.bin_min_points || h->tune.bin_min_points <= 0 || h->g.S < 1024 ||
-        v == sg::kStagedVariant)
+        v == sg::kStagedVariant || v == sg::kBlkVariant)
         return eval_one(h, a, x, P, o, nullptr, s);
     const int kb = sg::key_bits_for(P, h->g.d);
     const size_t ws_bytes = sg::sort_workspace_bytes(P, kb);
@@ -135,7 +135,7 @@ extern "C" {
 
 const char* sg_last_error(void) { return sg::last_error().c_str(); }
 const char* sg_status_string(int status) { return sg::status_name(status); }
-int sg_version(void) { return 115; }
+int sg_version(void) { return 200; }
 
 int sg_grid_create(int num_dims, int num_levels, const int* limits, sg_grid** out) {
     if (!out) return fail(sg::kInvalid, "sg_grid_create: out is NULL");
@@ -590,6 +590,26 @@ int sg_gather_probe_f32(const sg_grid* grid, const float* d_alpha, long long num
                                        d_out, as_stream(stream));
 }
 
+int sg_stage_probe_f64(const sg_grid* grid, const double* d_alpha, long long num_points,
+                       unsigned long long seed, double* d_out, void* stream) {
+    if (!grid || num_points < 0 || (num_points > 0 && (!d_alpha || !d_out)))
+        return fail(sg::kInvalid, "sg_stage_probe_f64: bad argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_stage_probe_f64(grid->g, grid->t, grid->tune.variant, d_alpha, num_points,
+                                      uint32_t(seed), d_out, as_stream(stream));
+}
+
+int sg_stage_probe_f32(const sg_grid* grid, const float* d_alpha, long long num_points,
+                       unsigned long long seed, float* d_out, void* stream) {
+    if (!grid || num_points < 0 || (num_points > 0 && (!d_alpha || !d_out)))
+        return fail(sg::kInvalid, "sg_stage_probe_f32: bad argument");
+    int st = check_device(grid);
+    if (st != sg::kOk) return st;
+    return sg::launch_stage_probe_f32(grid->g, grid->t, grid->tune32.variant, d_alpha,
+                                      num_points, uint32_t(seed), d_out, as_stream(stream));
+}
+
 int sg_points_to_row_major_f64(const double* d_in, int layout, int num_dims, long long num_points,
                                double* d_out, void* stream) {
     if (num_points < 0 || (num_points > 0 && (!d_in || !d_out)) || d_in == d_out)
@@ -622,6 +642,13 @@ int sg_batch_create(sg_grid* const* grids, int count, sg_batch** out) {
     std::vector<sg::Batch::GridRef> refs;
     for (int i = 0; i < count; ++i) {
         if (!grids[i]) return fail(sg::kInvalid, "sg_batch_create: NULL grid");
+        // the evaluate kernels address a grid's surpluses through a buffer of at most
+        // 2 GiB; FP64 batches check their members again at evaluate
+        if (grids[i]->g.N * int64_t(sizeof(float)) > int64_t(0x7fffffff))
+            return fail(sg::kRange, "sg_batch_create: grid " + std::to_string(i) +
+                                        "'s surpluses exceed 2 GiB");
+    }
+    for (int i = 0; i < count; ++i) {
         int st = check_device(grids[i]);           // uploads the grid's tables
         if (st != sg::kOk) return st;
         refs.push_back({&grids[i]->g, &grids[i]->t, grids[i]->tune.variant,
@@ -794,6 +821,11 @@ int sg_server_poll(sg_server* server, long long request_id, int* done) {
 int sg_server_wait(sg_server* server, long long request_id, double* latency_seconds) {
     if (!server) return fail(sg::kInvalid, "sg_server_wait: NULL server");
     return server->s.wait(request_id, latency_seconds);
+}
+
+int sg_server_inject_launch_failure(sg_server* server, int grid_id, int launches) {
+    if (!server) return fail(sg::kInvalid, "sg_server_inject_launch_failure: NULL server");
+    return server->s.inject_launch_failure(grid_id, launches);
 }
 
 int sg_server_stats(sg_server* server, long long* launches, long long* points_done,

@@ -29,6 +29,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "sg_grid.hpp"
 #include "sg_kernels.hpp"
@@ -132,9 +133,61 @@ using lds_byte = __attribute__((address_space(3))) uint8_t;
 // zero, expcnt (6:4) and lgkmcnt (11:8) at their maxima
 constexpr int kWaitVmcnt0 = 0x0F70;
 
+// Copy stage k into LDS, 16 bytes per lane and segment: each wave takes a contiguous
+// share of the stage's full segments (64 lanes, no lane mask) and of its partial ones,
+// and reads their descriptors four at a time (the descriptor table ends in four empty
+// entries).  Segment g is (source byte offset, LDS byte offset | lanes << kSegLanesShift).
+template <int TPB>
+__device__ __forceinline__ void copy_stage(__amdgpu_buffer_rsrc_t alpha, lds_byte* lds,
+                                           cptr<int32_t> seg_begin, cptr<int32_t> segs, int k) {
+    const int s0 = seg_begin[2 * k], sp = seg_begin[2 * k + 1], s1 = seg_begin[2 * k + 2];
+    constexpr int kWaves = TPB / 64;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    int per = (sp - s0 + kWaves - 1) / kWaves;
+    int g0 = s0 + wave * per;
+    int g1 = min(sp, g0 + per);
+    int g = g0;
+    for (; g + 4 <= g1; g += 4) {
+        int v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = segs[2 * g + i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                alpha, lds + uint32_t(v[2 * j + 1] & kSegLdsMask), 16, lane * 16u, v[2 * j], 0, 0);
+    }
+    for (; g < g1; ++g)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            alpha, lds + uint32_t(segs[2 * g + 1] & kSegLdsMask), 16, lane * 16u, segs[2 * g], 0, 0);
+    per = (s1 - sp + kWaves - 1) / kWaves;
+    g0 = sp + wave * per;
+    g1 = min(s1, g0 + per);
+    for (g = g0; g < g1; g += 4) {
+        int v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = segs[2 * g + i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (g + j < g1 && int(lane) < (v[2 * j + 1] >> kSegLanesShift))
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    alpha, lds + uint32_t(v[2 * j + 1] & kSegLdsMask), 16, lane * 16u, v[2 * j], 0, 0);
+        }
+    }
+}
+
+// a stage barrier: this wave's copies into LDS have landed (buffer loads to LDS count in
+// vmcnt, and the workgroup barrier's fence does not wait for them), then every wave of
+// the workgroup has arrived
+__device__ __forceinline__ void stage_barrier() {
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+    __syncthreads();
+}
+
 template <class T, class Acc, bool STG>
 struct State {
     static constexpr bool kStaged = STG;
+    static constexpr bool kBlk = false;
     __amdgpu_buffer_rsrc_t alpha;
     cptr<int32_t> leaf_base;
     cptr<uint8_t> bounds;
@@ -147,53 +200,8 @@ struct State {
     int stage, num_stages;
     int next_first;             // first leaf of the next stage
 
-    // copy stage k's blocks into LDS, 16 bytes per lane and segment: each wave takes a
-    // contiguous quarter of the stage's full segments (64 lanes, no lane mask) and of its
-    // partial ones, and reads their descriptors four at a time (the descriptor table
-    // ends in four empty entries)
     __device__ __forceinline__ void issue(int k) {
-        const int s0 = seg_begin[2 * k], sp = seg_begin[2 * k + 1], s1 = seg_begin[2 * k + 2];
-        constexpr int kWaves = kThreads / 64;      // staged launches use kThreads
-        const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-        const uint32_t lane = threadIdx.x & 63u;
-        int per = (sp - s0 + kWaves - 1) / kWaves;
-        int g0 = s0 + wave * per;
-        int g1 = min(sp, g0 + per);
-        int g = g0;
-        for (; g + 4 <= g1; g += 4) {
-            int v[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] = segs[2 * g + i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    alpha, lds + uint32_t(v[2 * j + 1] & 0xffff), 16, lane * 16u, v[2 * j], 0, 0);
-        }
-        for (; g < g1; ++g)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                alpha, lds + uint32_t(segs[2 * g + 1] & 0xffff), 16, lane * 16u, segs[2 * g], 0, 0);
-        per = (s1 - sp + kWaves - 1) / kWaves;
-        g0 = sp + wave * per;
-        g1 = min(s1, g0 + per);
-        for (g = g0; g < g1; g += 4) {
-            int v[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] = segs[2 * g + i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (g + j < g1 && int(lane) < (v[2 * j + 1] >> 16))
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                        alpha, lds + uint32_t(v[2 * j + 1] & 0xffff), 16, lane * 16u, v[2 * j],
-                        0, 0);
-            }
-        }
-    }
-    // a stage barrier: this wave's copies into LDS have landed (buffer loads to LDS
-    // count in vmcnt, and the workgroup barrier's fence does not wait for them), then
-    // every wave of the workgroup has arrived
-    __device__ __forceinline__ void stage_barrier() {
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        __syncthreads();
+        copy_stage<kThreads>(alpha, lds, seg_begin, segs, k);
     }
     // every wave is done with the current stage and the next one has landed: move on
     // and start copying the one after into the buffer just released
@@ -211,6 +219,56 @@ struct State {
     __device__ __forceinline__ T fetch_lds(int qq, uint32_t voff) const {
         return *reinterpret_cast<const __attribute__((address_space(3))) T*>(
             lds + (uint32_t(leaf_lds[qq]) + voff));
+    }
+    __device__ __forceinline__ T fetch_global(int qq, uint32_t voff) const {
+        return gather_leaf<T>(alpha, leaf_base[qq], voff);
+    }
+    __device__ __forceinline__ void leaf(T w, uint32_t voff) {
+        acc.add(w, fetch_global(q, voff));
+        ++q;
+    }
+};
+
+// State of the block-staged walk (grids whose walk invariant H = n - 1 - R holds, n - 1
+// = NM1 a compile-time constant).  Staged blocks follow each other in the current stage
+// buffer; cur is where the next one goes.  A block that does not fit what is left of the
+// buffer starts the next stage (the host planned the stages by the same rule).
+template <class T, class Acc, int NM1, int TPB, int STB>
+struct StateBlk {
+    static constexpr bool kStaged = true;
+    static constexpr bool kBlk = true;
+    static constexpr int kNM1 = NM1;
+    static constexpr int kStageBytes = STB;
+    __amdgpu_buffer_rsrc_t alpha;
+    cptr<int32_t> leaf_base;
+    cptr<uint8_t> bounds;
+    int q;                      // next leaf (uniform)
+    int bq;                     // unused (explicit sets do not take this walk)
+    Acc acc;
+    lds_byte* lds;              // two stage buffers of STB bytes
+    cptr<int32_t> seg_begin, segs;
+    uint32_t base, cur, limit;  // LDS byte offsets: current block, next block, buffer end
+    int stage, num_stages;
+
+    __device__ __forceinline__ void issue(int k) {
+        copy_stage<TPB>(alpha, lds, seg_begin, segs, k);
+    }
+    __device__ __forceinline__ void advance() {
+        stage_barrier();
+        ++stage;
+        cur = uint32_t(stage & 1) * uint32_t(STB);
+        limit = cur + uint32_t(STB);
+        if (stage + 1 < num_stages) issue(stage + 1);
+    }
+    template <uint32_t BYTES>
+    __device__ __forceinline__ void block_begin() {
+        acc.pin();
+        if (cur + BYTES > limit) advance();
+        base = cur;
+        cur += BYTES;
+    }
+    __device__ __forceinline__ T fetch_lds_at(uint32_t a) const {
+        return *reinterpret_cast<const __attribute__((address_space(3))) T*>(lds + a);
     }
     __device__ __forceinline__ T fetch_global(int qq, uint32_t voff) const {
         return gather_leaf<T>(alpha, leaf_base[qq], voff);
@@ -353,6 +411,62 @@ __device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, 
     }
 }
 
+// Block-staged static blocks: the outer level sum H is a compile-time constant, so every
+// shift of the in-block offset is one too, and the leaf's place in the block's LDS image
+// (blk_leaf_offset) becomes the LDS read's immediate offset.
+template <int J, int R, int c, int j, int SH, class Ln>
+__device__ __forceinline__ uint32_t leaf_offset_ct(const Ln& ln, uint32_t off) {
+    constexpr int l = simplex_level<J, R, c, j>();
+    off += ln.ub[j][l] << SH;
+    if constexpr (j > 0) return leaf_offset_ct<J, R, c, j - 1, SH + l>(ln, off);
+    return off;
+}
+
+template <int J, int R, int c>
+__host__ __device__ constexpr int64_t blk_leaf_lds(int H, int elem) {
+    int lv[J] = {};
+    for (int j = 0; j < J; ++j) lv[j] = Simplex<J, R>().l[c][j];
+    return blk_leaf_offset(J, lv, H, elem);
+}
+
+template <int J, int R, int H, int B, int c, int cend, class Ln, class T, class St>
+__device__ __forceinline__ void block_issue_blk(T (&a)[B], const Ln& ln, uint32_t vb, St& st) {
+    if constexpr (c < cend) {
+        constexpr uint32_t lo = uint32_t(blk_leaf_lds<J, R, c>(H, int(sizeof(T))));
+        a[c % B] = st.fetch_lds_at(leaf_offset_ct<J, R, c, J - 1, H>(ln, vb) + lo);
+        block_issue_blk<J, R, H, B, c + 1, cend>(a, ln, vb, st);
+    }
+}
+
+// With H a compile-time constant a leaf's address minus the block base depends on the
+// lane's tables alone, so the compiler would hoist the offsets of every leaf of every
+// block out of the walk and keep them all live.  An empty asm that claims to change the
+// offset tables at each block entry keeps that arithmetic inside the block.
+template <int J, int R, class Ln>
+__device__ __forceinline__ void launder_offsets(Ln& ln) {
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int l = 0; l <= R && l < int(sizeof(ln.ub[0]) / sizeof(ln.ub[0][0])); ++l)
+            asm volatile("" : "+v"(ln.ub[j][l]));
+}
+
+template <int J, int R, int H, int B, int c0, class Ln, class T, class St>
+__device__ __forceinline__ void block_batches_blk(const Ln& ln, T pp, uint32_t vb, St& st,
+                                                  T (&sn)[J]) {
+    constexpr int C = Simplex<J, R>::C;
+    if constexpr (c0 < C) {
+        constexpr int cend = (c0 + B < C) ? c0 + B : C;
+        T a[B];
+        block_issue_blk<J, R, H, B, c0, cend>(a, ln, vb, st);
+        if constexpr (kNestedSum<decltype(st.acc)>)
+            block_consume_nested<J, R, B, c0, cend>(a, sn, ln, pp, st.acc);
+        else
+            block_consume<J, R, B, c0, cend>(a, ln, pp, st.acc);
+        block_batches_blk<J, R, H, B, cend>(ln, pp, vb, st, sn);
+    }
+}
+
 // All C(R+J, J) leaves below a depth-J node whose remaining level budget R fits inside
 // the J tabled dimensions.  The staged walk reads a block from LDS when the host packed
 // it into a stage (the same rule, stage_whole, decides on both sides), else from global
@@ -361,7 +475,16 @@ template <int J, int R, int B, class Ln, class T, class St>
 __device__ __forceinline__ void static_block(const Ln& ln, T pp, uint32_t off, int hsum,
                                              St& st) {
     T sn[J];                   // nested partial sums (FP64 blocks)
-    if constexpr (St::kStaged) {
+    if constexpr (St::kBlk) {
+        constexpr int H = St::kNM1 - R;
+        if constexpr (H >= 0 && blk_bytes(J, R, H, int(sizeof(T))) <= St::kStageBytes) {
+            st.template block_begin<uint32_t(blk_bytes(J, R, H, int(sizeof(T))))>();
+            launder_offsets<J, R>(const_cast<Ln&>(ln));   // ln is the kernel's own lane
+            block_batches_blk<J, R, H, B, 0>(ln, pp, off + st.base, st, sn);
+        } else {
+            block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st, sn);
+        }
+    } else if constexpr (St::kStaged) {
         constexpr int kHmax = stage_hmax(J, R, int(sizeof(T)));
         if (hsum <= kHmax) {
             st.block_start(st.q);
@@ -416,6 +539,17 @@ template <int D, class T>
 constexpr int stage_batch() {
     return sizeof(T) == 8 && D >= 9 ? 4 : 8;
 }
+// batch of the block-staged walk
+#ifndef SG_BLK_BATCH64
+#define SG_BLK_BATCH64 4
+#endif
+#ifndef SG_BLK_BATCH32
+#define SG_BLK_BATCH32 8
+#endif
+template <int D, class T>
+constexpr int blk_batch() {
+    return sizeof(T) == 8 ? SG_BLK_BATCH64 : SG_BLK_BATCH32;
+}
 
 // Up to two dimensions above the tabled ones unroll this way: dimension K's levels
 // 0..R, each level unrolling dimension K-1 with the budget left, down to dimension J
@@ -462,6 +596,68 @@ constexpr int up2_max() {
     else return D == 10 || D == 12 ? 2 : D == 11 || D == 13 ? 1 : 0;
 }
 
+// Largest budget for which the general walk unrolls dimensions J+1 and J together when it
+// runs the block-staged blocks (FP32): 2 (the staged walk's 4 gains under 2 % over 2 and
+// doubles the kernels' code and build time).
+#ifndef SG_BLK_UP2_32
+#define SG_BLK_UP2_32 2
+#endif
+template <class T>
+constexpr int blk_up2_max() {
+    return sizeof(T) == 4 ? SG_BLK_UP2_32 : 0;
+}
+
+// The block-staged walk of a regular grid (every limit n - 1): every node below the
+// root admits levels 0..rem, and a depth-J node is always a whole static block.  Each
+// outer hat is computed from x at its level (y = x 2^l, t = floor(y), hat = 1 - |2(y - t)
+// - 1|) instead of incrementally, so no level loop carries hat state across the walk
+// below it: the live state per open level is its prefix product and offset.  For x on
+// the 2^-24 lattice both forms are exact, so the leaves' weights equal the other walks'.
+template <class T>
+__device__ __forceinline__ void hat_at(T x, int l, T& phi, uint32_t& t) {
+    const T y = ldexp(x, l);
+    const T fl = floor(y);
+    t = uint32_t(fl);
+    phi = T(1) - fabs(fma(T(2), y - fl, T(-1)));
+}
+
+// Lane state of the block-staged walk: the coordinates and the J tabled dimensions'
+// hat / offset tables.
+template <int D, int J, class T>
+struct LaneBlk {
+    T x[D];
+    T ph[J][8];
+    uint32_t ub[J][8];
+};
+
+// Which walk the block-staged kernels use.  FP64 takes the lean regular walk above
+// (hats computed at each level from x, nothing carried across a level loop: 158 VGPRs at
+// d = 10, no spills; unrolling any level above the blocks made the compiler spill).  FP32
+// has registers to spare and takes the general walk with its unrolled levels and
+// incremental hats (d = 10, n = 8, 2^24 points: 95.8 ms against 133 ms for the lean walk).
+template <class T>
+constexpr bool blk_lean_walk() {
+    return sizeof(T) == 8;
+}
+template <int K, int D, int J, int B, class T, class St>
+struct WalkerBlk {
+    using Ln = LaneBlk<D, J, T>;
+    static __device__ __forceinline__ void run(Ln& ln, T pp, uint32_t off, int hsum, int rem,
+                                               St& st) {
+        if constexpr (K == J - 1) {
+            static_block_dispatch<J, 8, B>(rem, ln, pp, off, hsum, st);
+        } else {
+            for (int l = 0; l <= rem; ++l) {
+                T phi;
+                uint32_t t;
+                hat_at(ln.x[K], l, phi, t);
+                const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
+                WalkerBlk<K - 1, D, J, B, T, St>::run(ln, pp * phi, o, hsum + l, rem - l, st);
+            }
+        }
+    }
+};
+
 // Depth-first walk.  Dimension K's level runs 0..bound, where bound = min(rem,
 // limit_K) for a restricted-regular set and comes from the node-code stream for an
 // explicit set.  Dimension d-1 is outermost, so for a fixed level sum the leaves come
@@ -488,7 +684,7 @@ struct Walker {
             }
             b = code & 0x7f;
         } else {
-            constexpr int kUp2 = up2_max<D, T, St::kStaged>();
+            constexpr int kUp2 = St::kBlk ? blk_up2_max<T>() : up2_max<D, T, St::kStaged>();
             if constexpr (J > 0 && K == J + 1 && K < D && kUp2 > 0) {
                 bool fits = rem <= kUp2 && rem <= lim[K] && rem <= lim[K - 1];
 #pragma unroll
@@ -535,9 +731,13 @@ struct Walker {
     }
 };
 
-template <int D, int LT, int J, int B, int W, bool EXPL, bool STG, class T, class Acc>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
+// NM1 >= 0: the block-staged walk of a restricted-regular grid with n - 1 = NM1, in
+// workgroups of TPB threads with two stage buffers of STB bytes.
+template <int D, int LT, int J, int B, int W, bool EXPL, bool STG, class T, class Acc,
+          int NM1 = -1, int TPB = kThreads, int STB = kStageBytes>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
 void k_eval(Params<T> prm) {
+    static_assert(NM1 < 0 || (STG && !EXPL), "the block-staged walk is a staged regular walk");
     int gi;
     int64_t first, count;
     if (prm.work) {
@@ -557,22 +757,28 @@ void k_eval(Params<T> prm) {
     // permutation, neighbouring lanes hold spatially close points, so a wave's gathers
     // share cache lines
     const int64_t pc = live ? (prm.perm ? int64_t(prm.perm[p]) : p) : first;
-    Lane<D, LT, J, T> ln;
+    constexpr bool kLean = NM1 >= 0 && blk_lean_walk<T>();
+    using Ln = std::conditional_t<kLean, LaneBlk<D, J, T>, Lane<D, LT, J, T>>;
+    Ln ln;
     const T* x = prm.coords + G->coord_off + pc * D;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         const T xk = x[k];
         if constexpr (J > 0) {
-            if (k < J) level_table<LT, T>(xk, ln.ph[k < J ? k : 0], ln.ub[k < J ? k : 0]);
+            if (k < J) level_table<8, T>(xk, ln.ph[k < J ? k : 0], ln.ub[k < J ? k : 0]);
         }
-        ln.m0[k] = T(2) * xk - T(1);
+        if constexpr (kLean) ln.x[k] = xk;
+        else ln.m0[k] = T(2) * xk - T(1);
     }
     int lim[D];
+    if constexpr (!kLean) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) lim[k] = G->limits[k];
+        for (int k = 0; k < D; ++k) lim[k] = G->limits[k];
+    }
     // buffer descriptor word 3 for gfx9-family raw (untyped) buffer access
     const T* alpha = prm.alpha ? prm.alpha + G->alpha_off : static_cast<const T*>(G->alpha_ptr);
-    using St = State<T, Acc, STG>;
+    using St = std::conditional_t<(NM1 >= 0), StateBlk<T, Acc, NM1, TPB, STB>,
+                                  State<T, Acc, STG>>;
     St st{};
     st.alpha = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(alpha), short(0),
                                                  int(G->N * int64_t(sizeof(T))), 0x00020000);
@@ -580,7 +786,23 @@ void k_eval(Params<T> prm) {
     st.bounds = as_const(G->bounds);
     st.q = 0;
     st.bq = 0;
-    if constexpr (STG) {
+    if constexpr (NM1 >= 0) {
+        __shared__ __attribute__((aligned(16))) uint8_t blk_buf[2 * STB];
+        const cptr<GridDesc::BlkStages> tb = &G->blk[sizeof(T) == 8 ? 0 : 1];
+        st.lds = (lds_byte*)blk_buf;
+        st.seg_begin = as_const(tb->seg_begin);
+        st.segs = as_const(tb->segs);
+        st.num_stages = tb->num_stages;
+        st.stage = 0;
+        st.base = 0;
+        st.cur = 0;
+        st.limit = uint32_t(STB);
+        if (st.num_stages > 0) {
+            st.issue(0);
+            stage_barrier();
+            if (st.num_stages > 1) st.issue(1);
+        }
+    } else if constexpr (STG) {
         __shared__ __attribute__((aligned(16))) uint8_t stage_buf[2 * kStageBytes];
         const cptr<GridDesc::Stages> tb = &G->stage[sizeof(T) == 8 ? 0 : 1];
         st.lds = (lds_byte*)stage_buf;
@@ -593,14 +815,102 @@ void k_eval(Params<T> prm) {
         st.next_first = -1;
         if (st.num_stages > 0) {
             st.issue(0);
-            st.stage_barrier();
+            stage_barrier();
             st.next_first = st.stage_first[1];
             if (st.num_stages > 1) st.issue(1);
         }
     }
-    Walker<D - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
+    if constexpr (kLean)
+        WalkerBlk<D - 1, D, J, B, T, St>::run(ln, T(1), 0u, 0, NM1, st);
+    else
+        Walker<D - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
     if (live) prm.out[G->out_off + pc] = st.acc.value();
 }
+
+// The staged-pattern probe: the data path of a staged walk without its arithmetic.  The
+// workgroup runs the walk's stage plan (the same buffer-to-LDS copies and stage
+// barriers); each lane visits the S leaves in walk order and reads one coefficient per
+// leaf at a random cell of the leaf's block, from LDS for staged leaves and by a buffer
+// load for the others, and accumulates it with one FMA.  No hats, no walk.  Its time is a
+// floor for the staged walk on the same grid: the walk does all of this and more.
+template <class T>
+struct ProbeParams {
+    const T* alpha;
+    int64_t N;
+    const int32_t* leaf_lds;     // [S] LDS byte offset (-1: read from global memory)
+    const int32_t* leaf_base;    // [S] base offsets in walk order
+    const uint8_t* leaf_lsum;    // [S] level sums in walk order
+    const int32_t* stage_first;  // [K+1]
+    const int32_t* seg_begin;
+    const int32_t* segs;
+    int num_stages;
+    int S;
+    int64_t P;
+    uint32_t seed;
+    T* out;
+};
+
+template <class T, int TPB, int STB>
+__global__ __launch_bounds__(TPB) void k_stage_probe(ProbeParams<T> p) {
+    __shared__ __attribute__((aligned(16))) uint8_t buf[2 * STB];
+    lds_byte* lds = (lds_byte*)buf;
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T*>(p.alpha), short(0), int(p.N * int64_t(sizeof(T))), 0x00020000);
+    const cptr<int32_t> leaf_lds = as_const(p.leaf_lds), leaf_base = as_const(p.leaf_base),
+                        stage_first = as_const(p.stage_first), seg_begin = as_const(p.seg_begin),
+                        segs = as_const(p.segs);
+    const cptr<uint8_t> lsum = as_const(p.leaf_lsum);
+    const int64_t pi = int64_t(blockIdx.x) * TPB + threadIdx.x;
+    uint32_t h = uint32_t(pi) * 0x9E3779B9u ^ p.seed;
+    h ^= h >> 15;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    const T w = T(1) + T(h & 1023u) * T(1.0 / 4096);
+    T acc = T(0);
+    const int K = p.num_stages;
+    int stage = 0, next = K > 0 ? stage_first[0] : p.S;
+    if (K > 0) {
+        copy_stage<TPB>(rsrc, lds, seg_begin, segs, 0);
+        stage_barrier();
+        if (K > 1) copy_stage<TPB>(rsrc, lds, seg_begin, segs, 1);
+        next = K > 1 ? stage_first[1] : p.S;
+    }
+    auto one = [&](int q) -> T {
+        const uint32_t cell = (h >> (q & 7)) & ((1u << lsum[q]) - 1u);
+        const int32_t at = leaf_lds[q];
+        if (at >= 0)
+            return *reinterpret_cast<const __attribute__((address_space(3))) T*>(
+                lds + (uint32_t(at) + cell * uint32_t(sizeof(T))));
+        return gather_leaf<T>(rsrc, leaf_base[q], cell * uint32_t(sizeof(T)));
+    };
+    int q = 0;
+    while (q < p.S) {
+        if (q == next) {                       // the walk reaches stage + 1
+            stage_barrier();
+            ++stage;
+            next = stage + 1 < K ? stage_first[stage + 1] : p.S;
+            if (stage + 1 < K) copy_stage<TPB>(rsrc, lds, seg_begin, segs, stage + 1);
+        }
+        const int qe = min(min(q + 8, next), p.S);
+        if (qe - q == 8) {
+            T a[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) a[i] = one(q + i);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc = fma(a[i], w, acc);
+        } else {
+            for (int i = q; i < qe; ++i) acc = fma(one(i), w, acc);
+        }
+        q = qe;
+    }
+    if (pi < p.P) p.out[pi] = acc;
+}
+
+// Launch the staged-pattern probe of grid g's staged walk in this precision (the
+// block-staged one where the grid runs it, else the staged one).
+template <class T>
+int launch_stage_probe(const Grid& g, const DeviceTables& t, int variant, const T* alpha,
+                       int64_t P, uint32_t seed, T* out, hipStream_t s);
 
 // Compiled variants of the register-table walk (LT = 8); the autotuner picks one per
 // grid shape and precision.  Tables for J dimensions cost 3 J LT registers (FP64);
@@ -609,9 +919,23 @@ struct Variant {
     int J, B, W;
     bool staged;
 };
+// waves per SIMD the block-staged walk's registers target, per element size (a multiple
+// of the workgroup's waves per SIMD, so whole workgroups fill a CU)
+#ifndef SG_BLK_WAVES64
+#define SG_BLK_WAVES64 3
+#endif
+#ifndef SG_BLK_WAVES32
+#define SG_BLK_WAVES32 3
+#endif
+constexpr int blk_waves(int elem) { return elem == 8 ? SG_BLK_WAVES64 : SG_BLK_WAVES32; }
+static_assert(blk_waves(8) % (blk_threads(8) / 256 > 0 ? blk_threads(8) / 256 : 1) == 0 &&
+              blk_waves(4) % (blk_threads(4) / 256 > 0 ? blk_threads(4) / 256 : 1) == 0,
+              "whole workgroups per CU");
 constexpr Variant kVariants[] = {{3, 8, 3, false}, {2, 8, 3, false}, {3, 4, 4, false},
-                                 {kStageJ, 0, 3, true}};   // batch: stage_batch()
+                                 {kStageJ, 0, 3, true},     // batch: stage_batch()
+                                 {kStageJ, 0, blk_waves(8), true}};   // waves: per precision
 static_assert(kStagedVariant == 3 && kVariants[kStagedVariant].staged, "staged variant index");
+static_assert(kBlkVariant == 4 && kVariants[kBlkVariant].staged, "block-staged variant index");
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // tabled dimensions the explicit-set kernels use (their node-code stream depends on it)
 constexpr int kExplicitJ = kExplicitTabDims;
@@ -653,6 +977,46 @@ hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int64_t blo
 #else
 ;
 #endif
+
+// The block-staged walk of a regular grid with n - 1 = nm1 (kBlkMinNM1 .. kBlkMaxNM1),
+// workgroups of blk_threads(sizeof(T)); built for the dimensions blk_built names.
+template <int D, class T, class Acc>
+hipError_t launch_blk(int nm1, const Params<T>& prm, int64_t blocks, hipStream_t s)
+#ifdef SG_EVAL_INSTANTIATE
+{
+    constexpr int Jd = kStageJ < D ? kStageJ : D;
+    constexpr int E = int(sizeof(T));
+    if constexpr (blk_built(D, E)) {
+        switch (nm1) {
+#define SG_NM(NN)                                                                          \
+    case NN:                                                                               \
+        hipLaunchKernelGGL((k_eval<D, 8, Jd, blk_batch<D, T>(), blk_waves(E), false, true, T, \
+                                   Acc, NN, blk_threads(E), blk_stage_bytes(E)>),           \
+                           dim3(unsigned(blocks)), dim3(blk_threads(E)), 0, s, prm);        \
+        return hipGetLastError();
+            SG_NM(2) SG_NM(3) SG_NM(4) SG_NM(5) SG_NM(6) SG_NM(7)
+#undef SG_NM
+            default: break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+#else
+;
+#endif
+
+template <class T, class Acc>
+hipError_t launch_d_blk(int d, int nm1, const Params<T>& prm, int64_t blocks, hipStream_t s) {
+    switch (d) {
+#define SG_D(DD) \
+    case DD: return launch_blk<DD, T, Acc>(nm1, prm, blocks, s);
+        SG_D(1) SG_D(2) SG_D(3) SG_D(4) SG_D(5) SG_D(6) SG_D(7) SG_D(8) SG_D(9) SG_D(10)
+        SG_D(11) SG_D(12) SG_D(13) SG_D(14) SG_D(15) SG_D(16) SG_D(17) SG_D(18) SG_D(19)
+        SG_D(20)
+#undef SG_D
+        default: return hipErrorInvalidValue;
+    }
+}
 
 template <bool EXPL, class T, class Acc>
 hipError_t launch_d(int d, int variant, int lt, const Params<T>& prm, int64_t blocks, int block,
@@ -701,7 +1065,8 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
         return kRange;
     }
     const int v = effective_variant(g, tu.variant, int(sizeof(T)));
-    const int block = kVariants[v].staged ? kThreads : tu.block;
+    const int block = v == kBlkVariant ? blk_threads(int(sizeof(T)))
+                      : kVariants[v].staged ? kThreads : tu.block;
     const int64_t blocks = (P + block - 1) / block;
     if (blocks > int64_t(0x7fffffff)) {
         set_error("evaluate: too many points for one launch");
@@ -715,12 +1080,62 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
     prm.work = nullptr;
     prm.perm = perm;
     prm.P = P;
-    const hipError_t e = g.explicit_set
+    const hipError_t e = v == kBlkVariant
+        ? launch_d_blk<T, Acc>(g.d, g.n - 1, prm, blocks, s)
+        : g.explicit_set
         ? launch_d<true, T, Acc>(g.d, v == kStagedVariant ? v : 0, table_levels(g, 0), prm,
                                  blocks, block, s)
         : launch_d<false, T, Acc>(g.d, v, table_levels(g, v), prm, blocks, block, s);
     if (e != hipSuccess) {
         set_error(std::string("k_eval launch: ") + hipGetErrorString(e));
+        return kHip;
+    }
+    return kOk;
+}
+
+template <class T>
+int launch_stage_probe(const Grid& g, const DeviceTables& t, int variant, const T* alpha,
+                       int64_t P, uint32_t seed, T* out, hipStream_t s) {
+    if (P == 0) return kOk;
+    const int e = sizeof(T) == 8 ? 0 : 1;
+    const int v = effective_variant(g, variant, int(sizeof(T)));
+    if (v != kBlkVariant && v != kStagedVariant) {
+        set_error("stage probe: the grid runs no staged walk in this precision");
+        return kUnsupported;
+    }
+    const GridDesc& hd = t.host_desc;
+    ProbeParams<T> p{};
+    p.alpha = alpha;
+    p.N = g.N;
+    p.leaf_base = hd.leaf_base;
+    p.leaf_lsum = hd.leaf_lsum;
+    p.S = int(g.S);
+    p.P = P;
+    p.seed = seed;
+    p.out = out;
+    hipError_t err;
+    if (v == kBlkVariant) {
+        p.leaf_lds = hd.blk[e].leaf_lds;
+        p.stage_first = hd.blk[e].stage_first;
+        p.seg_begin = hd.blk[e].seg_begin;
+        p.segs = hd.blk[e].segs;
+        p.num_stages = hd.blk[e].num_stages;
+        constexpr int TPB = blk_threads(int(sizeof(T)));
+        hipLaunchKernelGGL((k_stage_probe<T, TPB, blk_stage_bytes(int(sizeof(T)))>),
+                           dim3(unsigned((P + TPB - 1) / TPB)), dim3(TPB), 0, s, p);
+        err = hipGetLastError();
+    } else {
+        p.leaf_lds = hd.stage[e].leaf_lds;
+        p.stage_first = hd.stage[e].stage_first;
+        p.seg_begin = hd.stage[e].seg_begin;
+        p.segs = hd.stage[e].segs;
+        p.num_stages = hd.stage[e].num_stages;
+        hipLaunchKernelGGL((k_stage_probe<T, kThreads, kStageBytes>),
+                           dim3(unsigned((P + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, p);
+        err = hipGetLastError();
+    }
+    if (err != hipSuccess) {
+        set_error(std::string("stage probe launch: ") + hipGetErrorString(err));
         return kHip;
     }
     return kOk;

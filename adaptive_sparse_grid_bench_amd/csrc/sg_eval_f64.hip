@@ -9,6 +9,11 @@ int launch_evaluate_f64(const Grid& g, const DeviceTables& t, const double* alph
     return eval::evaluate_launch<double, eval::AccF64>(g, t, alpha, coords, P, out, tu, perm, s);
 }
 
+int launch_stage_probe_f64(const Grid& g, const DeviceTables& t, int variant, const double* alpha,
+                          int64_t P, uint32_t seed, double* out, hipStream_t s) {
+    return eval::launch_stage_probe<double>(g, t, variant, alpha, P, seed, out, s);
+}
+
 }  // namespace sg
 
 #include "../../include/sgrid.h"

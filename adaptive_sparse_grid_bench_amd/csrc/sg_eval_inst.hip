@@ -1,6 +1,7 @@
 // Instantiations of the evaluation walk for one precision (SG_PREC = 64 or 32) and one
 // range of dimensions (SG_DMIN..SG_DMAX); the Makefile compiles this file once per
-// range so the kernels build in parallel.
+// range (and once per dimension for the block-staged kernels) so the kernels build in
+// parallel.
 #define SG_EVAL_INSTANTIATE 1
 #include "sg_eval.hpp"
 
@@ -15,11 +16,18 @@ using TT = float;
 using AA = AccKahan;
 #endif
 
+// SG_BLK_ONLY: this object holds the block-staged kernels of the range (launch_blk),
+// otherwise the other walks (launch_variant), so the heavy kernels build in parallel.
+#ifdef SG_BLK_ONLY
+#define SG_INST(DD) \
+    template hipError_t launch_blk<DD, TT, AA>(int, const Params<TT>&, int64_t, hipStream_t);
+#else
 #define SG_INST(DD)                                                                        \
     template hipError_t launch_variant<DD, false, TT, AA>(int, int, const Params<TT>&,     \
                                                           int64_t, int, hipStream_t);      \
     template hipError_t launch_variant<DD, true, TT, AA>(int, int, const Params<TT>&,      \
                                                          int64_t, int, hipStream_t);
+#endif
 #define SG_IN_RANGE(DD) (SG_DMIN <= DD && DD <= SG_DMAX)
 
 #if SG_IN_RANGE(1)

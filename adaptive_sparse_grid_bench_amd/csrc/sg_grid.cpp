@@ -466,7 +466,7 @@ void build_stage_plan(const Grid& g, int J, const std::vector<std::array<int32_t
                 const int32_t lanes = (std::min(kStageSegBytes, b - o) + 15) / 16;
                 auto& to = lanes == kStageSegBytes / 16 ? full : part;
                 to.push_back(int32_t(int64_t(run_base) * elem + o));
-                to.push_back((run_at + o) | (lanes << 16));
+                to.push_back((run_at + o) | (lanes << kSegLanesShift));
             }
             used += (b + 15) & ~15;
             i = j;
@@ -503,6 +503,92 @@ void build_stage_plan(const Grid& g, int J, const std::vector<std::array<int32_t
     p.ok = true;
 }
 
+// The block-staged walk's copy plan (see BlockStagePlan): blocks in walk order, each
+// laid out by blk_leaf_offset; a stage takes the next staged blocks while they fit.  The
+// kernel applies the same greedy rule with the same compile-time block sizes, so the plan
+// needs no per-block table: only the copy segments of each stage.  Every leaf's LDS
+// offset is checked here against its HBM layout; a mismatch leaves the plan unusable.
+void build_blk_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 3>>& blocks,
+                    int elem, int stage_bytes, BlockStagePlan& p) {
+    p = BlockStagePlan();
+    if (g.explicit_set || g.N * elem > int64_t(0x7fffffff)) return;
+    const int nm1 = g.n - 1;
+    p.leaf_lds.assign(size_t(g.S), -1);
+    for (int j = 0; j < g.d; ++j)
+        if (g.limits[j] != nm1) return;                 // the walk assumes a regular grid
+    std::vector<int32_t> full, part;
+    int used = 0, buf = -1;
+    auto close_stage = [&]() {
+        if (buf < 0) return;
+        p.seg_begin.push_back(int32_t(p.segs.size() / 2));
+        p.segs.insert(p.segs.end(), full.begin(), full.end());
+        p.seg_begin.push_back(int32_t(p.segs.size() / 2));
+        p.segs.insert(p.segs.end(), part.begin(), part.end());
+        full.clear();
+        part.clear();
+        ++p.num_stages;
+    };
+    auto subspace_of = [&](int32_t base) -> int64_t {
+        const auto it = std::lower_bound(g.sub_base.begin(), g.sub_base.end(), base);
+        return (it == g.sub_base.end() || *it != base) ? -1 : int64_t(it - g.sub_base.begin());
+    };
+    for (const auto& blk : blocks) {
+        const int32_t q0 = blk[0], R = blk[1], H = blk[2];
+        if (H != nm1 - R) return;                       // not the regular-walk invariant
+        const int64_t bytes = blk_bytes(J, R, H, elem);
+        if (bytes > stage_bytes) continue;              // gathered from global memory
+        if (buf < 0 || used + bytes > stage_bytes) {
+            close_stage();
+            buf = buf < 0 ? 0 : buf ^ 1;
+            used = 0;
+            p.stage_first.push_back(q0);
+        }
+        const int32_t at = buf * stage_bytes + used;
+        // leaves of run s: (HBM base, LDS offset by the formula)
+        std::vector<std::vector<std::pair<int32_t, int64_t>>> runs(size_t(R) + 1);
+        const int64_t C = stage_binom(R + J, J);
+        int lv[kMaxDims] = {};
+        for (int64_t c = 0; c < C; ++c) {
+            const int32_t base = g.dfs_base[size_t(q0 + c)];
+            const int64_t id = subspace_of(base);
+            if (id < 0) return;
+            int s = 0;
+            for (int j = 0; j < J; ++j) {
+                lv[j] = g.sub_levels[size_t(id * g.d + j)];
+                s += lv[j];
+            }
+            if (s > R) return;
+            runs[size_t(s)].emplace_back(base, blk_leaf_offset(J, lv, H, elem));
+            p.leaf_lds[size_t(q0 + c)] = at + int32_t(blk_leaf_offset(J, lv, H, elem));
+        }
+        for (int s = 0; s <= R; ++s) {
+            auto& rl = runs[size_t(s)];
+            std::sort(rl.begin(), rl.end());
+            const int64_t sub_elems = int64_t(1) << (H + s);
+            const int64_t run_off = blk_run_offset(J, s, H, elem);
+            if (int64_t(rl.size()) != stage_binom(s + J - 1, J - 1)) return;
+            for (size_t i = 0; i < rl.size(); ++i) {
+                if (rl[i].first != rl[0].first + int64_t(i) * sub_elems) return;   // contiguous
+                if (rl[i].second != run_off + int64_t(i) * sub_elems * elem) return;
+            }
+            const int32_t len = int32_t(int64_t(rl.size()) * sub_elems * elem);
+            const int64_t src = int64_t(rl[0].first) * elem;
+            for (int32_t o = 0; o < len; o += kStageSegBytes) {
+                const int32_t lanes = (std::min(kStageSegBytes, len - o) + 15) / 16;
+                auto& to = lanes == kStageSegBytes / 16 ? full : part;
+                to.push_back(int32_t(src + o));
+                to.push_back((at + int32_t(run_off) + o) | (lanes << kSegLanesShift));
+            }
+        }
+        used += int(bytes);
+    }
+    close_stage();
+    p.stage_first.push_back(int32_t(g.S));
+    p.seg_begin.push_back(int32_t(p.segs.size() / 2));
+    p.segs.insert(p.segs.end(), 8, 0);                   // 4 empty segments of padding
+    p.ok = true;
+}
+
 }  // namespace
 
 int build_launch_tables(Grid& g, int hier_tile_values) {
@@ -522,6 +608,8 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
     // of the tabled dimensions below 8), packed into LDS stages
     g.stage[0] = StagePlan();
     g.stage[1] = StagePlan();
+    g.blk[0] = BlockStagePlan();
+    g.blk[1] = BlockStagePlan();
     {
         const int J = g.explicit_set ? explicit_tab_dims(g) : std::min(d, kStageJ);
         bool tabled = J > 0;
@@ -534,6 +622,10 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
             if (bw.ok && bw.nleaf == g.S) {
                 build_stage_plan(g, J, blocks, 8, g.stage[0]);
                 build_stage_plan(g, J, blocks, 4, g.stage[1]);
+                if (J == kStageJ) {
+                    build_blk_plan(g, J, blocks, 8, blk_stage_bytes(8), g.blk[0]);
+                    build_blk_plan(g, J, blocks, 4, blk_stage_bytes(4), g.blk[1]);
+                }
             }
         }
     }

@@ -44,6 +44,9 @@ constexpr int kExplicitTabDims = 3; // register-tabled dims of the explicit-set 
 // with others, stage_whole(); other leaves are gathered from global memory.
 constexpr int kStageBytes = 26624;   // 3 workgroups x 2 stages fill 156 KiB of a CU's 160
 constexpr int kStageSegBytes = 1024;
+// a copy segment's second word: LDS byte offset (below 2^24) | lanes << kSegLanesShift
+constexpr int kSegLanesShift = 24;
+constexpr int32_t kSegLdsMask = (1 << kSegLanesShift) - 1;
 constexpr int kStageJ = 3;
 
 SG_HOST_DEVICE constexpr int64_t stage_binom(int a, int b) {
@@ -69,6 +72,88 @@ SG_HOST_DEVICE constexpr int stage_hmax(int J, int R, int elem) {
         if (stage_whole(J, R, x, elem)) h = x;
     return h;
 }
+
+// Block-staged evaluation (the walk templated on n - 1 of a regular grid).  A
+// static block of budget R sits below outer levels summing to H = n - 1 - R, so on the
+// device its size and the LDS offset of each of its leaves are compile-time constants.
+// The blocks are laid out in LDS in walk order, one after another; inside a block the
+// leaves of level sum s (relative to the block) form one run, the runs s = 0..R follow
+// each other, each padded to 16 bytes, and inside a run the leaves keep their HBM order
+// (l_{J-1}, ..., l_1 ascending, the compact order of one level sum).  Every run is one
+// contiguous range in HBM as well.  A workgroup of blk_threads(elem) threads
+// double-buffers stages of blk_stage_bytes(elem); a stage takes the next blocks in walk
+// order while they fit.
+#ifndef SG_BLK_THREADS64
+#define SG_BLK_THREADS64 256
+#endif
+#ifndef SG_BLK_STAGE_BYTES64
+#define SG_BLK_STAGE_BYTES64 26624
+#endif
+#ifndef SG_BLK_THREADS32
+#define SG_BLK_THREADS32 256
+#endif
+#ifndef SG_BLK_STAGE_BYTES32
+#define SG_BLK_STAGE_BYTES32 26624
+#endif
+SG_HOST_DEVICE constexpr int blk_threads(int elem) {
+    return elem == 8 ? SG_BLK_THREADS64 : SG_BLK_THREADS32;
+}
+SG_HOST_DEVICE constexpr int blk_stage_bytes(int elem) {
+    return elem == 8 ? SG_BLK_STAGE_BYTES64 : SG_BLK_STAGE_BYTES32;
+}
+// the block-staged kernels are built for n - 1 in kBlkMinNM1..kBlkMaxNM1 (the tabled
+// dimensions hold levels 0..7) and for the dimensions where they beat the staged walk
+// (2^22-2^24 points, n = 5..8: FP64 from d = 12 on, up to 1.5x at d = 16..20; FP32 for
+// d = 8..16, up to 12 % at d = 10..12)
+constexpr int kBlkMinNM1 = 2;
+constexpr int kBlkMaxNM1 = 7;
+SG_HOST_DEVICE constexpr bool blk_built(int d, int elem) {
+    return elem == 8 ? (d >= 11 && d <= 20) : (d >= 7 && d <= 16);
+}
+
+// bytes of run s (level sum s inside the block) of a static block over J tabled
+// dimensions below outer level sum H: C(s+J-1, J-1) subspaces of 2^(H+s) elements
+SG_HOST_DEVICE constexpr int64_t blk_run_bytes(int J, int s, int H, int elem) {
+    return (stage_binom(s + J - 1, J - 1) * (int64_t(elem) << (H + s)) + 15) & ~int64_t(15);
+}
+SG_HOST_DEVICE constexpr int64_t blk_run_offset(int J, int s, int H, int elem) {
+    int64_t o = 0;
+    for (int r = 0; r < s; ++r) o += blk_run_bytes(J, r, H, elem);
+    return o;
+}
+SG_HOST_DEVICE constexpr int64_t blk_bytes(int J, int R, int H, int elem) {
+    return blk_run_offset(J, R + 1, H, elem);
+}
+// position of the leaf with levels lv[J-1..0] (sum s) inside its run: the number of
+// level vectors of sum s that precede it in the order (l_{J-1}, ..., l_1) ascending
+SG_HOST_DEVICE constexpr int64_t blk_run_pos(int J, const int* lv) {
+    int s = 0;
+    for (int j = 0; j < J; ++j) s += lv[j];
+    int64_t pos = 0;
+    int left = s;                       // sum still to place in dims j..0
+    for (int j = J - 1; j >= 1; --j) {
+        for (int a = 0; a < lv[j]; ++a) pos += stage_binom(left - a + j - 1, j - 1);
+        left -= lv[j];
+    }
+    return pos;
+}
+SG_HOST_DEVICE constexpr int64_t blk_leaf_offset(int J, const int* lv, int H, int elem) {
+    int s = 0;
+    for (int j = 0; j < J; ++j) s += lv[j];
+    return blk_run_offset(J, s, H, elem) + blk_run_pos(J, lv) * (int64_t(elem) << (H + s));
+}
+
+// How the block-staged walk copies one grid's blocks into LDS, for one element size.
+struct BlockStagePlan {
+    bool ok = false;
+    int num_stages = 0;
+    std::vector<int32_t> seg_begin;   // [2K+1] as StagePlan::seg_begin
+    std::vector<int32_t> segs;        // [nseg + 4][2] as StagePlan::segs
+    // for the staged-pattern probe (the kernel itself needs neither): LDS byte offset of
+    // every staged leaf in walk order (-1: gathered), first leaf of every stage ([K] = S)
+    std::vector<int32_t> leaf_lds;
+    std::vector<int32_t> stage_first;
+};
 
 struct HierTables {
     // One entry per tile / family / block of the pole-family decomposition of one
@@ -116,7 +201,8 @@ struct StagePlan {
     std::vector<int32_t> seg_begin;   // [2K+1] stage k's full segments (64 lanes) run
                                       //   from entry 2k to 2k+1, its partial ones to 2k+2
     std::vector<int32_t> segs;        // [nseg + 4][2]: source byte offset,
-                                      //   lds offset | lanes << 16 (4 zero entries at the end)
+                                      //   lds offset | lanes << kSegLanesShift (4 zero
+                                      //   entries at the end)
 };
 
 struct Grid {
@@ -140,6 +226,7 @@ struct Grid {
     std::vector<HierTables> hier;     // [d]
     std::vector<PairTables> hier2;    // [d/2]: dimensions (2p, 2p+1)
     StagePlan stage[2];               // [0]: 8-byte surpluses, [1]: 4-byte
+    BlockStagePlan blk[2];            // block-staged walk, [0]: 8-byte, [1]: 4-byte
 };
 
 // status codes shared with the C ABI

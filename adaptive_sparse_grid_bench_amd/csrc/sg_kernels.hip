@@ -858,9 +858,21 @@ int upload_tables(const Grid& g, DeviceTables& t) {
     }
     if ((st = upload(t.dfs_base, g.dfs_base.data(), g.dfs_base.size())) != kOk) return st;
     if ((st = upload(t.dfs_bound, g.dfs_bound.data(), g.dfs_bound.size())) != kOk) return st;
+    {
+        std::vector<uint8_t> ls(size_t(g.S), 0);
+        for (int64_t q = 0; q < g.S; ++q) {
+            const auto it = std::lower_bound(g.sub_base.begin(), g.sub_base.end(), g.dfs_base[q]);
+            const int64_t id = it - g.sub_base.begin();
+            int s = 0;
+            for (int j = 0; j < g.d; ++j) s += g.sub_levels[size_t(id * g.d + j)];
+            ls[size_t(q)] = uint8_t(s);
+        }
+        if ((st = upload(t.walk_lsum, ls.data(), ls.size())) != kOk) return st;
+    }
     eval::GridDesc desc{};
     desc.leaf_base = t.dfs_base;
     desc.bounds = t.dfs_bound;
+    desc.leaf_lsum = t.walk_lsum;
     desc.N = g.N;
     desc.n = g.n;
     for (int j = 0; j < g.d; ++j) desc.limits[j] = g.limits[j];
@@ -881,6 +893,21 @@ int upload_tables(const Grid& g, DeviceTables& t) {
         sd.seg_begin = sd.stage_first + p.stage_first.size();
         sd.segs = sd.seg_begin + p.seg_begin.size();
         sd.num_stages = int32_t(p.stage_first.size()) - 1;
+    }
+    for (int e = 0; e < 2; ++e) {
+        const BlockStagePlan& p = g.blk[e];
+        if (!p.ok) continue;
+        std::vector<int32_t> all(p.seg_begin);
+        all.insert(all.end(), p.segs.begin(), p.segs.end());
+        all.insert(all.end(), p.leaf_lds.begin(), p.leaf_lds.end());
+        all.insert(all.end(), p.stage_first.begin(), p.stage_first.end());
+        if ((st = upload(t.blk_tabs[e], all.data(), all.size())) != kOk) return st;
+        eval::GridDesc::BlkStages& sd = desc.blk[e];
+        sd.seg_begin = t.blk_tabs[e];
+        sd.segs = sd.seg_begin + p.seg_begin.size();
+        sd.leaf_lds = sd.segs + p.segs.size();
+        sd.stage_first = sd.leaf_lds + p.leaf_lds.size();
+        sd.num_stages = p.num_stages;
     }
     t.host_desc = desc;
     eval::GridDesc* ddesc = nullptr;
@@ -930,6 +957,9 @@ void free_tables(DeviceTables& t) {
     free_dev(t.dfs_bound);
     free_dev(t.stage_tabs[0]);
     free_dev(t.stage_tabs[1]);
+    free_dev(t.blk_tabs[0]);
+    free_dev(t.blk_tabs[1]);
+    free_dev(t.walk_lsum);
     if (t.desc) (void)hipFree(t.desc);
     t.desc = nullptr;
     for (auto& dm : t.dims) {

@@ -35,6 +35,17 @@ struct GridDesc {
         int32_t num_stages;
         int32_t pad;
     } stage[2];
+    // copy plans of the block-staged walk ([0]: 8-byte surpluses, [1]: 4-byte), see
+    // BlockStagePlan
+    struct BlkStages {
+        const int32_t* seg_begin;
+        const int32_t* segs;
+        const int32_t* leaf_lds;      // the staged-pattern probe's tables
+        const int32_t* stage_first;
+        int32_t num_stages;
+        int32_t pad;
+    } blk[2];
+    const uint8_t* leaf_lsum;     // level sum of every leaf in walk order (the probes)
 };
 
 // A run of at most one workgroup's points of one grid.
@@ -58,7 +69,9 @@ struct DeviceTables {
     uint8_t* sub_lsum = nullptr;    // [S] level sum of each subspace (compact order)
     int32_t* dfs_base = nullptr;    // [S]
     uint8_t* dfs_bound = nullptr;   // explicit sets: walk-node bounds
+    uint8_t* walk_lsum = nullptr;   // [S] level sum of each leaf in walk order
     int32_t* stage_tabs[2] = {};    // StagePlan arrays per element size, one allocation each
+    int32_t* blk_tabs[2] = {};      // BlockStagePlan arrays per element size
     void* desc = nullptr;           // the grid's evaluation descriptor (eval::GridDesc)
     eval::GridDesc host_desc{};     // host copy: tables set, launch offsets zero
     struct Dim {
@@ -114,18 +127,32 @@ struct EvalTuning {
 
 // the walk variant that reads surpluses from LDS stages
 constexpr int kStagedVariant = 3;
+// the block-staged walk (regular grids, n - 1 in kBlkMinNM1..kBlkMaxNM1)
+constexpr int kBlkVariant = 4;
 
 // default variants by dimension and precision, from timings at 2^22-2^24 points on
-// d = 5..16 and 20 (the autotuner refines them per grid shape)
+// d = 5..16 and 20 (the autotuner refines them per grid shape); a grid the block-staged
+// walk does not cover runs the staged walk
 inline int default_variant(int d, bool f32) {
-    if (f32) return kStagedVariant;
-    return d >= 17 ? 1 : kStagedVariant;
+    if (f32) return d >= 8 && d <= 16 ? kBlkVariant : kStagedVariant;
+    return d >= 12 ? kBlkVariant : kStagedVariant;   // falls back to the staged walk
 }
 // the variant a launch runs: the staged walk falls back to variant 0 for a grid whose
 // blocks cannot be staged in this element size
 inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
-    if (variant == kStagedVariant && !g.stage[elem_bytes == 8 ? 0 : 1].ok) return 0;
+    const int e = elem_bytes == 8 ? 0 : 1;
+    if (variant == kBlkVariant &&
+        (g.explicit_set || !g.blk[e].ok || g.n - 1 < kBlkMinNM1 || g.n - 1 > kBlkMaxNM1 ||
+         !blk_built(g.d, elem_bytes)))
+        variant = kStagedVariant;
+    if (variant == kStagedVariant && !g.stage[e].ok) return 0;
     return variant;
+}
+// the variant of a work-list launch (batches, the server): the block-staged walk runs
+// one grid per launch, so work lists take the staged walk in its place
+inline int work_list_variant(const Grid& g, int variant, int elem_bytes) {
+    const int v = effective_variant(g, variant, elem_bytes);
+    return v == kBlkVariant ? effective_variant(g, kStagedVariant, elem_bytes) : v;
 }
 
 // perm: lane order of the points (from sort_points) or null for the given order
@@ -194,5 +221,13 @@ int launch_gather_probe_f64(const Grid& g, const DeviceTables& t, const double* 
                             int64_t P, uint32_t seed, double* out, hipStream_t s);
 int launch_gather_probe_f32(const Grid& g, const DeviceTables& t, const float* alpha,
                             int64_t P, uint32_t seed, float* out, hipStream_t s);
+// The staged-pattern probe of the staged walk grid g runs under `variant` (block-staged or
+// staged): the walk's stage plan and one LDS read (global read for unstaged leaves) plus
+// one FMA per leaf at a random cell, no hats and no walk.  out[p] = point p's sum.
+int launch_stage_probe_f64(const Grid& g, const DeviceTables& t, int variant,
+                           const double* alpha, int64_t P, uint32_t seed, double* out,
+                           hipStream_t s);
+int launch_stage_probe_f32(const Grid& g, const DeviceTables& t, int variant, const float* alpha,
+                           int64_t P, uint32_t seed, float* out, hipStream_t s);
 
 }  // namespace sg

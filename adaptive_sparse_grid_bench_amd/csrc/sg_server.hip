@@ -83,6 +83,7 @@ struct Server::Impl {
     int64_t points_done = 0;
     bool stop = false;
     std::string error;
+    std::unordered_map<int, int> inject;   // grid -> launch groups still to fail (test hook)
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::thread worker;
@@ -190,8 +191,25 @@ struct Server::Impl {
         // staging and work lists, grouped by dimension count (outside the lock)
         std::vector<Request*> rs;
         for (int64_t id : batch) rs.push_back(&reqs[id]);
+        // injected failures: each pending one takes the next launch group of its grid
+        std::vector<char> inject_of(rs.size(), 0);
+        for (size_t i = 0; i < rs.size(); ++i) {
+            auto it = inject.find(rs[i]->grid);
+            if (it != inject.end() && it->second > 0) inject_of[i] = 1;
+        }
+        for (size_t i = 0; i < rs.size(); ++i) {
+            auto it = inject.find(rs[i]->grid);
+            if (inject_of[i] && it != inject.end() && it->second > 0) {
+                if (--it->second == 0) inject.erase(it);
+                for (size_t j = i + 1; j < rs.size(); ++j)   // one group per grid and launch
+                    if (rs[j]->grid == rs[i]->grid) inject_of[j] = 1;
+                break;
+            }
+        }
         lk.unlock();
-        std::map<std::tuple<int, int, int, int>, std::vector<eval::WorkItem>> groups;
+        // a grid with a pending injected failure gets a launch group of its own
+        std::map<std::tuple<int, int, int, int, int>, std::vector<eval::WorkItem>> groups;
+        std::map<std::tuple<int, int, int, int, int>, std::vector<size_t>> group_reqs;
         int64_t coff = 0, ooff = 0;
         std::vector<int64_t> out_off(rs.size());
         for (size_t i = 0; i < rs.size(); ++i) {
@@ -206,8 +224,10 @@ struct Server::Impl {
             D.alpha_ptr = ge.alpha;
             D.coord_off = coff;
             D.out_off = ooff;
-            auto& items = groups[std::make_tuple(g.explicit_set ? 1 : 0, g.d,
-                                                 eval::table_levels(g, 0), ge.variant)];
+            const auto key = std::make_tuple(g.explicit_set ? 1 : 0, g.d, eval::table_levels(g, 0),
+                                             ge.variant, inject_of[i] ? r.grid : -1);
+            auto& items = groups[key];
+            group_reqs[key].push_back(i);
             for (int64_t f = 0; f < r.P; f += eval::kThreads)
                 items.push_back({int32_t(i), int32_t(f),
                                  int32_t(std::min<int64_t>(eval::kThreads, r.P - f)), 0});
@@ -216,12 +236,19 @@ struct Server::Impl {
             ooff += r.P;
         }
         int64_t nitems = 0;
-        std::vector<std::tuple<int, int, int, int, int64_t, int64_t>> launches_v;
+        struct LaunchGroup {
+            int expl, d, lt, variant;
+            bool injected;
+            int64_t first, count;
+            const std::vector<size_t>* reqs;
+        };
+        std::vector<LaunchGroup> launches_v;
         for (auto& kv : groups) {
             std::memcpy(s.h_work + nitems, kv.second.data(), kv.second.size() * sizeof(eval::WorkItem));
-            launches_v.push_back(std::make_tuple(std::get<0>(kv.first), std::get<1>(kv.first),
-                                                 std::get<2>(kv.first), std::get<3>(kv.first),
-                                                 nitems, int64_t(kv.second.size())));
+            launches_v.push_back({std::get<0>(kv.first), std::get<1>(kv.first),
+                                  std::get<2>(kv.first), std::get<3>(kv.first),
+                                  std::get<4>(kv.first) >= 0, nitems, int64_t(kv.second.size()),
+                                  &group_reqs[kv.first]});
             nitems += int64_t(kv.second.size());
         }
         hipError_t e = hipMemcpyAsync(s.d_coords, s.h_coords, size_t(coff) * elem,
@@ -232,27 +259,66 @@ struct Server::Impl {
         if (e == hipSuccess)
             e = hipMemcpyAsync(s.d_work, s.h_work, size_t(nitems) * sizeof(eval::WorkItem),
                                hipMemcpyHostToDevice, s.stream);
-        for (auto& [expl, d, lt, variant, first, count] : launches_v) {
+        // A launch group that fails to launch fails its own requests only; the others'
+        // kernels are queued and complete as usual.
+        std::vector<char> failed(rs.size(), 0);
+        std::string launch_error;
+        for (const LaunchGroup& L : launches_v) {
             if (e != hipSuccess) break;
-            e = launch_group(expl != 0, d, lt, variant, first, count, s);
+            const hipError_t le = L.injected
+                ? hipErrorLaunchFailure
+                : launch_group(L.expl != 0, L.d, L.lt, L.variant, L.first, L.count, s);
+            if (le != hipSuccess) {
+                (void)hipGetLastError();               // clear the launch error
+                launch_error = std::string("server launch") + (L.injected ? " (injected)" : "") +
+                               ": " + hipGetErrorString(le);
+                for (size_t i : *L.reqs) failed[i] = 1;
+            }
         }
         if (e == hipSuccess)
             e = hipMemcpyAsync(s.h_out, s.d_out, size_t(ooff) * elem, hipMemcpyDeviceToHost,
                                s.stream);
         if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+        const auto now = std::chrono::steady_clock::now();
         lk.lock();
-        s.out_off = out_off;
         ++launches;
         if (e != hipSuccess) {
             error = std::string("server launch: ") + hipGetErrorString(e);
             (void)hipStreamSynchronize(s.stream);
             for (Request* r : rs) {
                 r->status = kHip;
+                r->t_done = now;
                 budget_used -= r->bytes;
             }
             s.reqs.clear();
+            s.out_off.clear();
             s.busy = false;
             cv_done.notify_all();
+            return true;
+        }
+        s.reqs.clear();
+        s.out_off.clear();
+        bool any_failed = false;
+        for (size_t i = 0; i < rs.size(); ++i) {
+            if (failed[i]) {
+                rs[i]->status = kHip;
+                rs[i]->t_done = now;
+                budget_used -= rs[i]->bytes;
+                any_failed = true;
+            } else {
+                s.reqs.push_back(rs[i]->id);
+                s.out_off.push_back(out_off[i]);
+            }
+        }
+        if (any_failed) {
+            error = launch_error;
+            cv_done.notify_all();
+        }
+        if (s.reqs.empty()) {                      // nothing ran: free the slot
+            lk.unlock();
+            (void)hipStreamSynchronize(s.stream);
+            lk.lock();
+            s.busy = false;
         }
         return true;
     }
@@ -336,7 +402,12 @@ int Server::add_grid(const Grid* g, const DeviceTables* t, const void* alpha, in
         set_error("server: grid lives on another device");
         return kInvalid;
     }
-    p->grids.push_back({g, t, alpha, effective_variant(*g, variant, int(p->elem))});
+    // the evaluate kernels address one grid's surpluses through a buffer of at most 2 GiB
+    if (g->N * int64_t(p->elem) > int64_t(0x7fffffff)) {
+        set_error("server: the grid's surpluses exceed 2 GiB in this precision");
+        return kRange;
+    }
+    p->grids.push_back({g, t, alpha, work_list_variant(*g, variant, int(p->elem))});
     *id = int(p->grids.size()) - 1;
     return kOk;
 }
@@ -397,6 +468,17 @@ int Server::poll(int64_t id, int* done) {
         return kInvalid;
     }
     *done = it->second.status <= 0 ? 1 : 0;
+    return kOk;
+}
+
+int Server::inject_launch_failure(int grid, int count) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (grid < 0 || grid >= int(p->grids.size()) || count < 0) {
+        set_error("server: bad grid or count");
+        return kInvalid;
+    }
+    if (count == 0) p->inject.erase(grid);
+    else p->inject[grid] = count;
     return kOk;
 }
 

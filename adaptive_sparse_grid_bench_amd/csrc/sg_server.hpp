@@ -27,6 +27,8 @@ class Server {
     int wait(int64_t id, double* latency_s);
     int poll(int64_t id, int* done);
     int stats(int64_t* launches, int64_t* points, int64_t* waiting, int64_t* budget_used);
+    // test hook: the next `count` launch groups holding grid's requests fail to launch
+    int inject_launch_failure(int grid, int count);
 
   private:
     struct Impl;

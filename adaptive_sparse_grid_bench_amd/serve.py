@@ -10,7 +10,7 @@ log-uniform).  Arrivals are Poisson at a rate set as a fraction of the capacity 
 by a closed-loop run.  Latency = completion - arrival (queueing included); throughput =
 points / (last completion - first arrival).
 
-Usage: python -m adaptive_sparse_grid_bench_amd.serve [--grids 64] [--requests 600]
+Usage: python -m adaptive_sparse_grid_bench_amd.serve [--grids 64] [--requests 5000]
        [--grid-dir DIR]   (resident grids from compact-grid files, see sgrid.save/load)
 """
 from __future__ import annotations
@@ -50,10 +50,12 @@ def save_grids(grids, grid_dir: str) -> None:
 
 
 def make_grids(count: int = 64, seed: int = 21, adaptive_every: int = 4,
-               max_points: int = 1 << 25, grid_dir: str | None = None):
+               max_points: int = 1 << 25, grid_dir: str | None = None,
+               keep_nodal: bool = False):
     """Resident grids with FP64 surpluses and FP32 copies (torch CUDA tensors).  With
     grid_dir, grids are read from its compact-grid files when present and written there
-    after being built otherwise."""
+    after being built otherwise.  keep_nodal also keeps each grid's nodal values (a host
+    array, "nodal") for checking the hierarchization."""
     import torch
     if grid_dir and os.path.isdir(grid_dir) and any(f.endswith(".sgrid") for f in os.listdir(grid_dir)):
         return load_grids(grid_dir)
@@ -81,8 +83,11 @@ def make_grids(count: int = 64, seed: int = 21, adaptive_every: int = 4,
         else:
             a = torch.exp(-((x - centre) ** 2).sum(1) / (2 * 0.15 ** 2))
         del x
+        nodal = a.cpu().numpy() if keep_nodal else None
         g.hierarchize(a)
         grids.append({"grid": g, "a64": a, "a32": a.float(), "kind": kind})
+        if keep_nodal:
+            grids[-1]["nodal"] = nodal
     torch.cuda.synchronize()
     if grid_dir:
         save_grids(grids, grid_dir)
@@ -170,41 +175,58 @@ def run_plain_f64(grids, reqs, arrivals=None):
             "p50_s": p50, "p99_s": p99, "requests": len(reqs)}, outs
 
 
-def experiment(grids, n_requests: int = 600, load: float = 0.8, seed: int = 5):
-    """Closed-loop capacity, then Poisson arrivals at `load` x capacity for the FP32
-    Kahan server, the FP64 server and the plain FP64 launches."""
+def experiment(grids, n_requests: int = 5000, loads=(0.5, 0.8), seed: int = 5,
+               keep_outputs: int = 0):
+    """Closed-loop capacity of the FP32 Kahan server, then for each load Poisson arrivals
+    at load x capacity for the FP32 Kahan server, the FP64 server and the plain FP64
+    launches (the same requests and arrival times for all three).  With keep_outputs,
+    the first keep_outputs requests and their outputs of the last load are returned too
+    (res["_kept"]) for checking."""
     reqs = make_requests(grids, n_requests, seed)
     run_server(grids, reqs[: min(50, len(reqs))], "f32c")          # warm-up
     cap, _ = run_server(grids, reqs, "f32c")
-    rate = load * cap["requests"] / cap["seconds"]                   # requests / s
-    rng = np.random.default_rng(seed + 1)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
-    f32, o32 = run_server(grids, reqs, "f32c", arrivals)
-    f64, o64 = run_server(grids, reqs, "f64", arrivals)
-    plain, op = run_plain_f64(grids, reqs, arrivals)
-    err = max(float(np.max(np.abs(a.astype(np.float64) - b))) if len(a) else 0.0
-              for a, b in zip(o32, o64))
-    same = all(np.array_equal(a, b) for a, b in zip(o64, op))
-    return {"capacity_closed_loop": cap, "arrival_rate_per_s": rate, "load": load,
+    cap_f64, _ = run_server(grids, reqs, "f64")
+    res = {"requests": len(reqs), "points": int(sum(len(p) for _, p, _ in reqs)),
+           "capacity_closed_loop": {"f32c_server": cap, "f64_server": cap_f64},
+           "loads": [],
+           "grids": {"count": len(grids),
+                     "adaptive": sum(g["kind"] == "adaptive" for g in grids),
+                     "grid_points": int(sum(g["grid"].N for g in grids))}}
+    for li, load in enumerate(loads):
+        rate = load * cap["requests"] / cap["seconds"]               # requests / s
+        rng = np.random.default_rng(seed + 1 + li)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+        f32, o32 = run_server(grids, reqs, "f32c", arrivals)
+        f64, o64 = run_server(grids, reqs, "f64", arrivals)
+        plain, op = run_plain_f64(grids, reqs, arrivals)
+        err = [np.abs(a.astype(np.float64) - b) for a, b in zip(o32, o64) if len(a)]
+        allerr = np.concatenate(err) if err else np.zeros(1)
+        same = all(np.array_equal(a, b) for a, b in zip(o64, op))
+        res["loads"].append({
+            "load": load, "arrival_rate_per_s": rate,
             "f32c_server": f32, "f64_server": f64, "plain_f64": plain,
-            "f32c_max_abs_err_vs_f64": err, "f64_server_equals_plain": same,
-            "grids": {"count": len(grids),
-                      "adaptive": sum(g["kind"] == "adaptive" for g in grids),
-                      "grid_points": int(sum(g["grid"].N for g in grids))}}
+            "f32c_vs_f64_server": {"max_abs": float(allerr.max()),
+                                   "rms": float(np.sqrt(np.mean(allerr ** 2)))},
+            "f64_server_equals_plain": bool(same),
+            "speedup_f32c_server_vs_plain_f64": plain["seconds"] / f32["seconds"],
+            "speedup_f64_server_vs_plain_f64": plain["seconds"] / f64["seconds"]})
+        if keep_outputs and li == len(loads) - 1:
+            res["_kept"] = [(reqs[i], o64[i], o32[i]) for i in range(min(keep_outputs, len(reqs)))]
+    return res
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grids", type=int, default=64)
-    ap.add_argument("--requests", type=int, default=600)
-    ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--requests", type=int, default=5000)
+    ap.add_argument("--loads", default="0.5,0.8")
     ap.add_argument("--grid-dir", default=None,
                     help="read the resident grids from compact-grid files here (written on "
                          "the first run)")
     args = ap.parse_args()
     t = time.time()
     grids = make_grids(args.grids, grid_dir=args.grid_dir)
-    res = experiment(grids, args.requests, args.load)
+    res = experiment(grids, args.requests, tuple(float(x) for x in args.loads.split(",")))
     res["setup_s"] = time.time() - t
     print(json.dumps(res), flush=True)
 

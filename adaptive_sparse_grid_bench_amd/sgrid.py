@@ -40,14 +40,15 @@ EXPORTS = [
     "sg_hierarchize_f64", "sg_hierarchize_f32",
     "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
     "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
-    "sg_gather_probe_f64", "sg_gather_probe_f32", "sg_sample_f64", "sg_sample_f32",
+    "sg_gather_probe_f64", "sg_gather_probe_f32", "sg_stage_probe_f64", "sg_stage_probe_f32",
+    "sg_sample_f64", "sg_sample_f32",
     "sg_file_write", "sg_file_info", "sg_file_read",
     "sg_points_to_row_major_f64", "sg_points_to_row_major_f32",
     "sg_batch_create", "sg_batch_destroy", "sg_batch_values", "sg_batch_set_points",
     "sg_batch_hierarchize_f64", "sg_batch_hierarchize_f32", "sg_batch_evaluate_f64",
     "sg_batch_evaluate_f32c", "sg_batch_run_graph_f64", "sg_batch_run_graph_f32c",
     "sg_server_create", "sg_server_destroy", "sg_server_add_grid", "sg_server_submit",
-    "sg_server_poll", "sg_server_wait", "sg_server_stats",
+    "sg_server_poll", "sg_server_wait", "sg_server_stats", "sg_server_inject_launch_failure",
 ]
 
 
@@ -61,8 +62,9 @@ class Tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-EVAL_VARIANTS = 4
+EVAL_VARIANTS = 5
 STAGED_VARIANT = 3          # surplus blocks staged in LDS by the workgroup
+BLK_VARIANT = 4             # block-staged walk (restricted-regular grids, compile-time n)
 
 
 class ServerConfig(C.Structure):
@@ -129,6 +131,8 @@ def _load() -> C.CDLL:
         "sg_sample_f32": (i, [vp, i, C.POINTER(C.c_double), ll, vp, vp]),
         "sg_gather_probe_f64": (i, [vp, vp, ll, C.c_ulonglong, vp, vp]),
         "sg_gather_probe_f32": (i, [vp, vp, ll, C.c_ulonglong, vp, vp]),
+        "sg_stage_probe_f64": (i, [vp, vp, ll, C.c_ulonglong, vp, vp]),
+        "sg_stage_probe_f32": (i, [vp, vp, ll, C.c_ulonglong, vp, vp]),
         "sg_batch_create": (i, [C.POINTER(vp), i, C.POINTER(vp)]),
         "sg_batch_destroy": (i, [vp]),
         "sg_batch_values": (i, [vp, llp, llp]),
@@ -146,6 +150,7 @@ def _load() -> C.CDLL:
         "sg_server_poll": (i, [vp, ll, ip]),
         "sg_server_wait": (i, [vp, ll, C.POINTER(C.c_double)]),
         "sg_server_stats": (i, [vp, llp, llp, llp, llp]),
+        "sg_server_inject_launch_failure": (i, [vp, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -173,6 +178,40 @@ def _ptr(t) -> int:
     if isinstance(t, int):
         return t
     return t.data_ptr()
+
+
+def _fp_dtype(t, name: str):
+    """float64 or float32 dtype of tensor t (ValueError for anything else)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype not in (torch.float64, torch.float32):
+        raise ValueError(f"{name}: dtype {t.dtype}; float64 or float32 expected")
+    return t.dtype
+
+
+def _check(t, name: str, dtype, numel=None, shape=None, min_numel=None) -> None:
+    """Validate a tensor handed to the library: dtype, C-contiguity, size.  Raises
+    ValueError.  _on_device checks the devices once every tensor has passed this."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous (row-major)")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: {t.numel()} elements, expected {numel}")
+    if min_numel is not None and t.numel() < min_numel:
+        raise ValueError(f"{name}: {t.numel()} elements, at least {min_numel} expected")
+
+
+def _on_device(**tensors) -> None:
+    for name, t in tensors.items():
+        if not t.is_cuda:
+            raise ValueError(f"{name}: must be a CUDA (device) tensor")
 
 
 def _stream(stream) -> int:
@@ -232,6 +271,23 @@ class Grid:
                                  lidx.ctypes.data_as(C.POINTER(C.c_longlong))))
         return dp.reshape(self.d, self.n), lidx
 
+    @property
+    def is_explicit(self) -> bool:
+        """True for an explicit (refined-subspace) level set: it has no dp matrix."""
+        dp = np.zeros(max(self.d * self.n, 1), np.int64)
+        st = lib.sg_grid_tables(self._h, dp.ctypes.data_as(C.POINTER(C.c_longlong)), None)
+        if st == SG_E_UNSUPPORTED:
+            return True
+        check(st)
+        return False
+
+    @property
+    def limits(self) -> np.ndarray:
+        """Largest level of each dimension in the level set (the restriction vector of a
+        restricted-regular grid, asg.c:82-83)."""
+        lv, _ = self.subspaces()
+        return lv.max(axis=0).astype(np.int32)
+
     def subspaces(self):
         lv = np.zeros((self.S, self.d), np.uint8)
         base = np.zeros(self.S, np.int64)
@@ -289,19 +345,28 @@ class Grid:
                                 _stream(stream)))
 
     def grid_coords(self, coords, stream=None) -> None:
+        import torch
+        _check(coords, "coords", torch.float64, numel=self.N * self.d)
+        _on_device(coords=coords)
         check(lib.sg_grid_coords_f64(self._h, _ptr(coords), _stream(stream)))
 
     def hierarchize(self, values, stream=None) -> None:
         """In place; FP64 or FP32 by the tensor's dtype (asg.c:557-571)."""
         import torch
-        fn = lib.sg_hierarchize_f64 if values.dtype == torch.float64 else lib.sg_hierarchize_f32
+        dt = _fp_dtype(values, "values")
+        _check(values, "values", dt, numel=self.N)
+        _on_device(values=values)
+        fn = lib.sg_hierarchize_f64 if dt == torch.float64 else lib.sg_hierarchize_f32
         check(fn(self._h, _ptr(values), _stream(stream)))
 
     def sample(self, function: str, out, params=None, stream=None) -> None:
         """out[o] = function at grid point o, on the device (asg.c:143-152).  function is
         a SAMPLE_FUNCTIONS name; params as include/sgrid.h states (a flat float sequence)."""
         import torch
-        fn = lib.sg_sample_f64 if out.dtype == torch.float64 else lib.sg_sample_f32
+        dt = _fp_dtype(out, "out")
+        _check(out, "out", dt, numel=self.N)
+        _on_device(out=out)
+        fn = lib.sg_sample_f64 if dt == torch.float64 else lib.sg_sample_f32
         p = np.ascontiguousarray(params if params is not None else [], np.float64)
         check(fn(self._h, SAMPLE_FUNCTIONS[function],
                  p.ctypes.data_as(C.POINTER(C.c_double)) if p.size else None, p.size,
@@ -314,8 +379,15 @@ class Grid:
         mode "ref": the reference's operation order, bit-identical to evaluate0.
         """
         import torch
+        dt = _fp_dtype(alpha, "alpha")
+        _check(alpha, "alpha", dt, numel=self.N)
+        if not hasattr(coords, "shape") or coords.dim() != 2:
+            raise ValueError("coords: expected a [P][d] tensor")
         P = coords.shape[0]
-        f64 = alpha.dtype == torch.float64
+        _check(coords, "coords", dt, shape=(P, self.d))
+        _check(out, "out", dt, min_numel=P)
+        _on_device(alpha=alpha, coords=coords, out=out)
+        f64 = dt == torch.float64
         if mode == "fast":
             fn = lib.sg_evaluate_f64 if f64 else lib.sg_evaluate_f32c
         elif mode == "ref":
@@ -362,7 +434,24 @@ def gather_probe(grid: "Grid", alpha, out, seed: int = 1, stream=None) -> None:
     in-block gather per subspace per point, no hats): the evaluate kernels' gather
     roofline.  out[p] = sum of point p's gathered values."""
     import torch
-    fn = lib.sg_gather_probe_f64 if alpha.dtype == torch.float64 else lib.sg_gather_probe_f32
+    dt = _fp_dtype(alpha, "alpha")
+    _check(alpha, "alpha", dt, numel=grid.N)
+    _check(out, "out", dt)
+    _on_device(alpha=alpha, out=out)
+    fn = lib.sg_gather_probe_f64 if dt == torch.float64 else lib.sg_gather_probe_f32
+    check(fn(grid._h, _ptr(alpha), out.numel(), seed, _ptr(out), _stream(stream)))
+
+
+def stage_probe(grid: "Grid", alpha, out, seed: int = 1, stream=None) -> None:
+    """The staged-pattern probe of the staged walk grid runs in alpha's precision, for
+    out.numel() points: the walk's stage copies and one read (LDS or global) plus one FMA
+    per subspace per point, no hats and no walk; a floor for that walk's time."""
+    import torch
+    dt = _fp_dtype(alpha, "alpha")
+    _check(alpha, "alpha", dt, numel=grid.N)
+    _check(out, "out", dt)
+    _on_device(alpha=alpha, out=out)
+    fn = lib.sg_stage_probe_f64 if dt == torch.float64 else lib.sg_stage_probe_f32
     check(fn(grid._h, _ptr(alpha), out.numel(), seed, _ptr(out), _stream(stream)))
 
 
@@ -381,11 +470,18 @@ def points_to_row_major(src, layout: str, d: int, num_points: int, out, stream=N
 def uniform_points(out, seed: int = 42, stream=None) -> None:
     """Fill a float64/float32 tensor with k * 2^-24 values (splitmix64 stream `seed`)."""
     import torch
-    fn = lib.sg_uniform_points_f64 if out.dtype == torch.float64 else lib.sg_uniform_points_f32
+    dt = _fp_dtype(out, "out")
+    _check(out, "out", dt)
+    _on_device(out=out)
+    fn = lib.sg_uniform_points_f64 if dt == torch.float64 else lib.sg_uniform_points_f32
     check(fn(out.numel(), seed, _ptr(out), _stream(stream)))
 
 
 def to_f32(src, dst, stream=None) -> None:
+    import torch
+    _check(src, "src", torch.float64)
+    _check(dst, "dst", torch.float32, min_numel=src.numel())
+    _on_device(src=src, dst=dst)
     check(lib.sg_convert_f64_to_f32(_ptr(src), _ptr(dst), src.numel(), _stream(stream)))
 
 
@@ -433,22 +529,37 @@ class Batch:
                                       co.ctypes.data_as(llp)))
         self.point_offsets, self.coord_offsets = po, co
 
+    def _check_eval(self, alpha, coords, out):
+        if self.point_offsets is None:
+            raise ValueError("set_points() first")
+        dt = _fp_dtype(alpha, "alpha")
+        _check(alpha, "alpha", dt, numel=self.total_values)
+        _check(coords, "coords", dt, numel=int(self.coord_offsets[-1]))
+        _check(out, "out", dt, min_numel=int(self.point_offsets[-1]))
+        _on_device(alpha=alpha, coords=coords, out=out)
+        return dt
+
     def hierarchize(self, values, stream=None):
         import torch
-        fn = lib.sg_batch_hierarchize_f64 if values.dtype == torch.float64 else \
+        dt = _fp_dtype(values, "values")
+        _check(values, "values", dt, numel=self.total_values)
+        _on_device(values=values)
+        fn = lib.sg_batch_hierarchize_f64 if dt == torch.float64 else \
             lib.sg_batch_hierarchize_f32
         check(fn(self._h, _ptr(values), _stream(stream)))
 
     def evaluate(self, alpha, coords, out, stream=None):
         import torch
-        fn = lib.sg_batch_evaluate_f64 if alpha.dtype == torch.float64 else \
+        dt = self._check_eval(alpha, coords, out)
+        fn = lib.sg_batch_evaluate_f64 if dt == torch.float64 else \
             lib.sg_batch_evaluate_f32c
         check(fn(self._h, _ptr(alpha), _ptr(coords), _ptr(out), _stream(stream)))
 
     def run_graph(self, alpha, coords, out, hierarchize=True, stream=None):
         """Hierarchize (in place) then evaluate as one hipGraph launch."""
         import torch
-        fn = lib.sg_batch_run_graph_f64 if alpha.dtype == torch.float64 else \
+        dt = self._check_eval(alpha, coords, out)
+        fn = lib.sg_batch_run_graph_f64 if dt == torch.float64 else \
             lib.sg_batch_run_graph_f32c
         check(fn(self._h, _ptr(alpha), _ptr(coords), _ptr(out), int(hierarchize),
                  _stream(stream)))
@@ -512,6 +623,11 @@ class Server:
         finally:
             self._keep.pop(rid, None)
         return lat.value
+
+    def inject_launch_failure(self, grid_id: int, launches: int = 1) -> None:
+        """Test hook: the next `launches` launch groups holding requests of grid_id fail to
+        launch (their requests raise SG_E_HIP at wait); other requests are unaffected."""
+        check(lib.sg_server_inject_launch_failure(self._h, grid_id, launches))
 
     def stats(self) -> dict:
         v = [C.c_longlong() for _ in range(4)]

@@ -8,10 +8,20 @@ One step = restore the nodal values into the grid buffer, hierarchize them (in p
 evaluate the interpolant at the P points.  Inputs are resident in HBM before the timed
 region.  ``value`` is evaluated points per second over the whole timed steps;
 hierarchized grid points per second and the per-kernel times come from HIP events
-recorded on the stream the kernels run on.
+recorded on the stream the kernels run on.  The reference's own throughput convention
+(HI_FLOPS = 3 d N per hierarchization, EV_FLOPS = P S (11 d + 2) per evaluation,
+adaptive_sparse_grid_bench.c:48-52, printed as GFLOPS at :1795) is reported beside it.
 
-Prints one JSON line (see the fields below).  ``--gpus`` above 1 is refused: this build
-runs on one GPU.
+The evaluate kernel's roofline is the staged-pattern probe (sg_stage_probe_*): the same
+stage plan, copies and barriers as the walk, one read per subspace per point from LDS
+(global memory for the blocks the walk does not stage) and one FMA, no hats and no walk.
+The walk does all of that work and more, so probe time / walk time is at most 1.
+
+The CPU leg (rank 0 only) times the reference's own OpenMP/SSE path (omp_evaluate5 and
+omp_hierarchize3, compiled from its source into oracle/_ref/) on a bounded sample, and
+checks samples of every timed GPU output against the CPU oracle (max and RMS errors).
+
+Prints one JSON line.  ``--gpus`` above 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
 
@@ -26,63 +36,65 @@ sys.path.insert(0, ROOT)
 
 METRIC = "eval pts/s & hierarchize GP/s at d=10,n=8; served p50/p99 vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 VARIANT_NAMES = {0: "gather, 3 tabled dims, batches of 8", 1: "gather, 2 tabled dims",
                  2: "gather, batches of 4, 4 waves/SIMD",
-                 3: "staged in LDS, 3 tabled dims, FP64 batches of 4 from d=9 (8 below), "
-                    "dimension 3 unrolled below small budgets"}
+                 3: "staged in LDS, 3 tabled dims, leaves at table-given LDS offsets",
+                 4: "block-staged in LDS, compile-time n, leaves at compile-time LDS offsets"}
+SAMPLE = 4096          # points of each timed output checked against the CPU oracle
 
 
-def cpu_baseline(d: int, n: int, P_sample: int, seed: int) -> dict:
-    """The reference's own OpenMP/SSE CPU path (omp_evaluate5 and omp_hierarchize3 of
-    adaptive_sparse_grid_bench.c, compiled from source into oracle/_ref/), timed on this
-    host's cores on a bounded sample of the same workload; the oracle's FP64 restatement
-    of evaluate0 if that library is absent."""
-    import numpy as np
-    from adaptive_sparse_grid_bench_amd import workloads as W
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    pts = W.uniform_points(P_sample, d, seed=seed)
+def _cpu_model() -> str:
     try:
-        from oracle import refbind
-        if not refbind.available():
-            raise ImportError("oracle/_ref/libasgref.so not built")
-        g = refbind.RefGrid(d, n)
-        x = pts.astype(np.float32)
-        vals = g.get_values()
-        hier = []
-        for _ in range(5):
-            g.set_values(vals)
-            t0 = time.perf_counter()
-            g.hierarchize("omp_hierarchize3")
-            hier.append(time.perf_counter() - t0)
-        g.evaluate(x[: min(P_sample, 256)], "omp_evaluate5")          # warm-up
-        t0 = time.perf_counter()
-        g.evaluate(x, "omp_evaluate5")
-        dt = time.perf_counter() - t0
-        return {"value": P_sample / dt, "unit": "points/s", "cores": threads,
-                "kind": "reference",
-                "sample": f"reference omp_evaluate5 (FP32, as the reference computes) on "
-                          f"{P_sample} of the workload's query points at d={d} n={n}; "
-                          f"omp_hierarchize3 median of 5 on the full grid",
-                "hierarchize_gp_per_s": g.N / sorted(hier)[2],
-                "host_cpu": _cpu_model()}
-    except (ImportError, OSError) as exc:
-        from oracle import oracle as O
-        sh = O.tables(d, n)
-        alpha = O.hierarchize(sh, W.f_gauss(O.grid_coords(sh)))
-        t0 = time.perf_counter()
-        O.evaluate(sh, alpha, pts)
-        dt = time.perf_counter() - t0
-        return {"value": P_sample / dt, "unit": "points/s", "cores": threads, "kind": "port",
-                "sample": f"oracle restatement of evaluate0 (FP64) on {P_sample} points "
-                          f"at d={d} n={n} ({exc})", "host_cpu": _cpu_model()}
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def precision_form(args) -> dict:
+def _host(t, k=None):
+    import torch
+    torch.cuda.synchronize()
+    return (t if k is None else t[:k]).cpu().numpy()
+
+
+def _timed(torch, fn, stream, reps=3):
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        ev[0].record(stream)
+        fn()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        best = min(best, ev[0].elapsed_time(ev[1]))
+    return best
+
+
+def _profile_json(d, n, P, variant):
+    """rocprofv3 figures of the evaluate kernel recorded in profiles/ (HBM bytes from
+    FETCH_SIZE / WRITE_SIZE scaled by a plain copy; unit counters), if they are for this
+    workload and walk variant."""
+    path = os.path.join(ROOT, "profiles", f"eval_d{d}n{n}_traffic.json")
+    try:
+        tj = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if tj.get("num_points") != P or tj.get("variant") != variant:
+        return None
+    return tj
+
+
+# --------------------------------------------------------------------- forms --
+
+def precision_form(args, stream):
     """d=20, n=6 (N=1,515,009): a seeded sum of 8 anisotropic Gaussians (centres U[0.2,
     0.8]^20, widths log-uniform [0.05, 0.5]) hierarchized in FP64 on the GPU, evaluated at
-    2^24 uniform points in FP32 with Kahan accumulation and in FP64; max and RMS error of
-    the FP32 result against the FP64 walk (which the tests hold to the CPU FP64 oracle
-    within 1e-12 relative to sum |terms|)."""
+    2^24 uniform points in FP32 with Kahan accumulation and in FP64.  Returns the
+    timings and a sample for the CPU oracle check."""
     import torch
     from adaptive_sparse_grid_bench_amd import sgrid
     from adaptive_sparse_grid_bench_amd import workloads as W
@@ -101,90 +113,206 @@ def precision_form(args) -> dict:
     pts = torch.empty((P, d), dtype=torch.float64, device="cuda")
     sgrid.uniform_points(pts, seed=11)
     o64 = torch.empty(P, dtype=torch.float64, device="cuda")
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ev[0].record()
-    g.evaluate(f, pts, o64)
-    ev[1].record()
-    a32, p32 = f.float(), pts.float()
+    t64 = _timed(torch, lambda: g.evaluate(f, pts, o64, stream), stream, reps=2)
+    a32 = f.float()
+    sample = {"d": d, "n": n, "alpha": _host(f), "points": _host(pts, args.precision_sample),
+              "f64": _host(o64, args.precision_sample)}
+    p32 = pts.float()
     del pts
     o32 = torch.empty(P, dtype=torch.float32, device="cuda")
-    ev[2].record()
-    g.evaluate(a32, p32, o32)
-    ev[3].record()
-    torch.cuda.synchronize()
-    err = (o32.double() - o64).abs()
-    umax = o64.abs().max().item()
+    t32 = _timed(torch, lambda: g.evaluate(a32, p32, o32, stream), stream, reps=2)
+    sample["f32c"] = _host(o32, args.precision_sample)
+    v64, v32 = g.get_tuning()["eval_variant_f64"], g.get_tuning()["eval_variant_f32"]
     res = {"workload": f"d=20 n=6 (N={g.N}, S={g.S}), sum of 8 Gaussians, 2^{args.log2_points} "
-                       "points; FP32 Kahan vs FP64 walk",
-           "f64_ms": ev[0].elapsed_time(ev[1]), "f32c_ms": ev[2].elapsed_time(ev[3]),
-           "max_abs_err": err.max().item(), "rms_err": err.pow(2).mean().sqrt().item(),
-           "max_abs_u": umax}
-    res["max_rel_to_max_u"] = res["max_abs_err"] / umax if umax > 0 else None
-    del o64, o32, a32, p32, f, err
+                       "points; FP64 walk and FP32 Kahan walk",
+           "f64_ms": t64, "f32c_ms": t32, "f64_variant": v64, "f32c_variant": v32,
+           "f64_pts_per_s": P / (t64 * 1e-3), "f32c_pts_per_s": P / (t32 * 1e-3),
+           "max_abs_u": o64.abs().max().item()}
+    del o64, o32, a32, p32, f
     torch.cuda.empty_cache()
-    return res
+    return res, sample
 
 
-def other_forms(g, alpha, pts, out64, P, args) -> dict:
-    """FP32-Kahan evaluation of the same workload, the batched form (256 grids, one
-    hipGraph launch vs per-grid launches) and the served form (continuous batching vs
-    plain FP64 per-request launches, p50/p99)."""
+def other_forms(g, alpha, pts, P, args, stream):
+    """FP32-Kahan evaluation of the headline workload, the precision form, the batched form
+    (256 grids, one hipGraph launch vs per-grid launches) and the served form (continuous
+    batching vs plain FP64 per-request launches at two loads, p50/p99).  Returns the
+    results and the samples the CPU leg checks."""
     import torch
     from adaptive_sparse_grid_bench_amd import batched, serve
-    res = {}
+    res, samples = {}, {}
     a32, p32 = alpha.float(), pts.float()
-    o32 = torch.empty(P, dtype=torch.float32, device=out64.device)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    g.evaluate(a32, p32, o32)
-    torch.cuda.synchronize()
-    ev[0].record()
-    g.evaluate(a32, p32, o32)
-    ev[1].record()
-    torch.cuda.synchronize()
-    ms = ev[0].elapsed_time(ev[1])
-    err = (o32.double() - out64).abs()
+    o32 = torch.empty(P, dtype=torch.float32, device="cuda")
+    ms = _timed(torch, lambda: g.evaluate(a32, p32, o32, stream), stream, reps=2)
     res["f32c"] = {"evaluate_ms": ms, "pts_per_s": P / (ms * 1e-3),
-                   "max_abs_err_vs_f64": err.max().item(),
-                   "rms_err_vs_f64": err.pow(2).mean().sqrt().item()}
+                   "variant": g.get_tuning()["eval_variant_f32"]}
+    samples["f32c"] = _host(o32, SAMPLE)
     del a32, p32, o32
-    res["precision"] = precision_form(args)
+    res["precision"], samples["precision"] = precision_form(args, stream)
     parts = batched.build(256, tune=True)
     res["batched"] = batched.measure(*parts)
-    del parts
+    specs, grids, b, nodal, coords = parts
+    # every grid's first 16 points after a graph run in each precision
+    bs = {"grids": [(gr.d, gr.n) for gr in grids], "k": 16}
+    for prec, dt in (("f64", torch.float64), ("f32c", torch.float32)):
+        a = nodal.to(dt).clone()
+        x = coords.to(dt)
+        out = torch.empty(int(b.point_offsets[-1]), dtype=dt, device="cuda")
+        b.run_graph(a, x, out, hierarchize=True)
+        a_h, x_h, o_h = _host(a), _host(x), _host(out)
+        bs[prec] = [(a_h[int(b.value_offsets[i]):int(b.value_offsets[i + 1])],
+                     x_h[int(b.coord_offsets[i]):int(b.coord_offsets[i]) + 16 * gr.d]
+                     .reshape(16, gr.d),
+                     o_h[int(b.point_offsets[i]):int(b.point_offsets[i]) + 16])
+                    for i, gr in enumerate(grids)]
+        del a, x, out
+    samples["batched"] = bs
+    del parts, specs, grids, b, nodal, coords
     grids = serve.make_grids(64)
-    res["served"] = serve.experiment(grids, args.served_requests, 0.8)
+    sv = serve.experiment(grids, args.served_requests, (0.5, 0.8), keep_outputs=48)
+    kept = sv.pop("_kept")
+    samples["served"] = {"requests": kept,
+                         "grids": [(gr["grid"].subspaces()[0] if gr["grid"].is_explicit else None,
+                                    gr["grid"].d, gr["grid"].n, _host(gr["a64"]), _host(gr["a32"]))
+                                   for gr in grids]}
+    res["served"] = sv
     del grids
     torch.cuda.empty_cache()
-    return res
+    return res, samples
 
 
-def _cpu_model() -> str:
+# --------------------------------------------------------------------- CPU leg --
+
+def _errs(out, ref, absref=None, u=None):
+    import numpy as np
+    out = np.asarray(out, np.float64)
+    e = np.abs(out - ref)
+    r = {"max_abs": float(e.max()), "rms": float(np.sqrt(np.mean(e * e))), "points": int(e.size)}
+    if absref is not None:
+        r["max_rel_to_sum_abs_terms"] = float(np.max(e / np.maximum(absref, 1e-300)))
+    if u is not None:
+        r["max_rel_to_abs_u"] = float(np.max(e / np.maximum(np.abs(u), 1e-300)))
+    return r
+
+
+def cpu_leg(d, n, P_sample, seed, head, samples):
+    """The reference's OpenMP/SSE CPU path timed on this host, and the CPU oracle's check
+    of the timed GPU outputs (the oracle is used here only as the checker)."""
+    import numpy as np
+    from adaptive_sparse_grid_bench_amd import workloads as W
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    baseline, reason = None, None
     try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+        from oracle import refbind
+        if not refbind.available():
+            raise ImportError("oracle/_ref/libasgref.so is not built")
+        g = refbind.RefGrid(d, n)
+        threads = g.omp_threads()
+        pts = W.uniform_points(P_sample, d, seed=seed)
+        x = pts.astype(np.float32)
+        vals = g.get_values()
+        hier = []
+        for _ in range(5):
+            g.set_values(vals)
+            t0 = time.perf_counter()
+            g.hierarchize("omp_hierarchize3")
+            hier.append(time.perf_counter() - t0)
+        g.evaluate(x[:256], "omp_evaluate5")                       # warm-up
+        t0 = time.perf_counter()
+        g.evaluate(x, "omp_evaluate5")
+        dt = time.perf_counter() - t0
+        baseline = {"value": P_sample / dt, "unit": "points/s", "cores": threads,
+                    "kind": "reference",
+                    "sample": f"reference omp_evaluate5 (FP32, as the reference computes) on "
+                              f"{P_sample} of the workload's query points at d={d} n={n} "
+                              f"({dt:.1f} s); omp_hierarchize3 median of 5 on the full grid",
+                    "omp_threads": threads, "host_cores_available": cores,
+                    "hierarchize_gp_per_s": g.N / sorted(hier)[2],
+                    "evaluate_gflops_ref_convention": P_sample * g.S * (11 * d + 2) / dt / 1e9,
+                    "host_cpu": _cpu_model()}
+    except (ImportError, OSError) as exc:
+        reason = f"reference CPU library unavailable: {exc}"
+    checks = {}
+    sh = O.tables(d, n)
+    alpha_ref = O.hierarchize(sh, head["nodal"])
+    checks["hierarchize_bit_exact"] = bool(np.array_equal(alpha_ref, head["alpha"]))
+    ref = O.evaluate(sh, alpha_ref, head["points"])
+    absref = O.evaluate(sh, np.abs(alpha_ref), head["points"])
+    checks["f64"] = _errs(head["f64"], ref, absref, ref)
+    if "f32c" in samples:
+        checks["f32c"] = _errs(samples["f32c"], ref, absref, ref)
+    if "precision" in samples:
+        s = samples["precision"]
+        sp = O.tables(s["d"], s["n"])
+        r = O.evaluate(sp, s["alpha"], s["points"])
+        ar = O.evaluate(sp, np.abs(s["alpha"]), s["points"])
+        checks["precision"] = {"f64": _errs(s["f64"], r, ar, r), "f32c": _errs(s["f32c"], r, ar, r)}
+    if "batched" in samples:
+        s = samples["batched"]
+        out = {}
+        for prec in ("f64", "f32c"):
+            es, rs, worst = [], [], 0.0
+            for (dd, nn), (a, x, o) in zip(s["grids"], s[prec]):
+                sb = O.tables(dd, nn)
+                r = O.evaluate(sb, a.astype(np.float64), x.astype(np.float64))
+                ar = O.evaluate(sb, np.abs(a.astype(np.float64)), x.astype(np.float64))
+                e = np.abs(o.astype(np.float64) - r)
+                es.append(e)
+                worst = max(worst, float(np.max(e / np.maximum(ar, 1e-300))))
+            e = np.concatenate(es)
+            out[prec] = {"max_abs": float(e.max()), "rms": float(np.sqrt(np.mean(e * e))),
+                         "max_rel_to_sum_abs_terms": worst, "points": int(e.size),
+                         "note": "FP32 outputs against the FP64 oracle on the FP32 surpluses"
+                         if prec == "f32c" else "FP64 outputs against the FP64 oracle"}
+        checks["batched"] = out
+    if "served" in samples:
+        s = samples["served"]
+        shapes = {}
+        e64, e32 = [], []
+        for (gi, p64, _), o64, o32 in s["requests"]:
+            k = min(16, len(p64))
+            if k == 0:
+                continue
+            lv, dd, nn, a64, _ = s["grids"][gi]
+            if gi not in shapes:
+                shapes[gi] = O.xshape(lv) if lv is not None else O.tables(dd, nn)
+            ev = O.x_evaluate if lv is not None else O.evaluate
+            r = ev(shapes[gi], a64, p64[:k])
+            e64.append(np.abs(o64[:k] - r))
+            e32.append(np.abs(o32[:k].astype(np.float64) - r))
+        for name, es in (("f64_server", e64), ("f32c_server", e32)):
+            e = np.concatenate(es)
+            checks[f"served_{name}"] = {"max_abs": float(e.max()),
+                                        "rms": float(np.sqrt(np.mean(e * e))),
+                                        "points": int(e.size), "requests": len(es)}
+    checks["oracle"] = "CPU oracle (oracle/sg_oracle.c, FP64 restatement of evaluate0)"
+    return baseline, reason, checks
 
+
+# ------------------------------------------------------------------------ main --
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--d", type=int, default=10)
     ap.add_argument("--n", type=int, default=8)
     ap.add_argument("--log2-points", type=int, default=24)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 16)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 19)
+    ap.add_argument("--precision-sample", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the FP32, batched and served forms (headline FP64 line only)")
-    ap.add_argument("--served-requests", type=int, default=600)
+    ap.add_argument("--served-requests", type=int, default=5000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="only run the headline evaluate and its staged-pattern probe "
+                         "(for profiling both kernels)")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -220,7 +348,18 @@ def main() -> int:
     pts = torch.empty((P, d), dtype=torch.float64, device=dev)
     sgrid.uniform_points(pts, seed=args.seed)
     out = torch.empty(P, dtype=torch.float64, device=dev)
+    variant = g.get_tuning()["eval_variant_f64"]
+    probe_out = torch.empty(P, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+
+    if args.probe_only:
+        alpha.copy_(nodal)
+        g.hierarchize(alpha, stream)
+        g.evaluate(alpha, pts, out, stream)
+        sgrid.stage_probe(g, alpha, probe_out, seed=7, stream=stream)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe_only": True, "variant": variant}), flush=True)
+        return 0
 
     def step(events=None):
         alpha.copy_(nodal)
@@ -250,40 +389,41 @@ def main() -> int:
     hier_med = hier_ms[len(hier_ms) // 2]
     ms_per_step = 1e3 * wall / args.steps
     value = P * args.steps / wall
+    head = {"nodal": _host(nodal), "alpha": _host(alpha), "points": _host(pts, SAMPLE),
+            "f64": _host(out, SAMPLE)}
 
-    # algorithmic bytes of the evaluate kernel: one coefficient gather per subspace,
-    # the point's coordinates and its output value
-    bytes_per_point = g.S * 8 + d * 8 + 8
-    achieved = P * bytes_per_point / (eval_avg * 1e-3) / 1e9
-    # HBM bytes per launch of this kernel, measured with rocprofv3 counters
-    # (FETCH_SIZE scaled by a plain copy's reading, plus WRITE_SIZE)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"eval_d{d}n{n}_traffic.json")
-    if os.path.exists(tpath):
-        try:
-            tj = json.load(open(tpath))
-            if tj.get("num_points") == P:
-                traffic = tj.get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
-    variant = g.get_tuning()["eval_variant_f64"]
-
-    # achievable gather rate of this grid's shape: one random in-block 8-byte gather per
-    # subspace per point, blocks in level-sum order, no hats (the gather roofline)
-    probe_out = torch.empty(P, dtype=torch.float64, device=dev)
-    pev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    probe_ms = []
-    for _ in range(3):
-        pev[0].record(stream)
-        sgrid.gather_probe(g, alpha, probe_out, seed=7, stream=stream)
-        pev[1].record(stream)
-        torch.cuda.synchronize()
-        probe_ms.append(pev[0].elapsed_time(pev[1]))
+    # the roofline: the staged-pattern probe of the walk this grid runs (its floor)
+    probe_ms = _timed(torch, lambda: sgrid.stage_probe(g, alpha, probe_out, seed=7,
+                                                       stream=stream), stream, reps=3)
     del probe_out
-    probe_best = min(probe_ms[1:])
-    gather_bytes = P * g.S * 8
-    achievable = gather_bytes / (probe_best * 1e-3) / 1e9
+    gather_bytes = P * g.S * 8                   # one 8-byte read per subspace per point
+    prof = _profile_json(d, n, P, variant)
+    traffic = prof.get("hbm_bytes_per_launch") if prof else None
+    roofline = {
+        "kernel": f"k_eval<{d},...,double> walk variant {variant} ({VARIANT_NAMES[variant]})",
+        "bound": "lds-staged",
+        "achieved": gather_bytes / (eval_avg * 1e-3) / 1e9,
+        "peak": gather_bytes / (probe_ms * 1e-3) / 1e9,
+        "unit": "GB/s",
+        "frac": probe_ms / eval_avg,
+        "traffic": traffic,
+        "ceiling": "k_stage_probe: the walk's stage plan, copies and barriers, one read per "
+                   "subspace per point (LDS, or global memory for unstaged blocks) and one "
+                   "FMA; no hats, no walk",
+        "probe_ms": probe_ms,
+        "evaluate_ms": eval_avg,
+        "bytes_per_point_read": g.S * 8,
+        "algorithmic_bytes_per_point": g.S * 8 + d * 8 + 8,
+        "hbm": {"achieved": traffic / (eval_avg * 1e-3) / 1e9 if traffic else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": traffic / (eval_avg * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                "note": "rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel, scaled by a plain "
+                        "copy's reading (profiles/)"},
+        "units": prof.get("units") if prof else None,
+    }
 
+    ev_flops = P * g.S * (11 * d + 2)
+    hi_flops = 3 * d * g.N
     rec = {
         "metric": METRIC,
         "value": value,
@@ -306,44 +446,41 @@ def main() -> int:
         },
         "hierarchize_gp_per_s": g.N / (hier_med * 1e-3),
         "hierarchize_ms": hier_med,
-        # algorithmic bytes of a full hierarchization: every value read and written once
-        # per dimension sweep (2 d N 8 bytes); the pair kernel moves each value once per
-        # two dimensions, so its HBM bytes are about half of these
+        # a full hierarchization reads and writes every value once per dimension sweep;
+        # the pair kernel sweeps two dimensions per pass, so d/2 passes move 2 N 8 bytes
+        # each: these are the HBM bytes (the grid is far larger than the LDS)
         "hierarchize_roofline": {
             "kernels": "k_hierarchize_pair (two dimensions per launch), k_hierarchize_dim",
             "bound": "hbm",
-            "algorithmic_bytes": 2 * d * g.N * 8,
-            "achieved": 2 * d * g.N * 8 / (hier_med * 1e-3) / 1e9,
+            "bytes": d * g.N * 8,
+            "achieved": d * g.N * 8 / (hier_med * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": 2 * d * g.N * 8 / (hier_med * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "frac": d * g.N * 8 / (hier_med * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "evaluate_ms": eval_avg,
         "evaluate_pts_per_s_kernel": P / (eval_avg * 1e-3),
-        "roofline": {
-            "kernel": f"k_eval<{d},...,double> walk variant {variant} ({VARIANT_NAMES[variant]})",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_point": bytes_per_point,
-            "achievable_gather": {
-                "GBps": achievable,
-                "probe_ms": probe_best,
-                "evaluate_gather_GBps": gather_bytes / (eval_avg * 1e-3) / 1e9,
-                "frac": probe_best / eval_avg,
-                "pattern": "sg_gather_probe_f64: per point one random 8-byte gather inside "
-                           "every subspace block, level-sum order, 8 in flight per lane",
-            },
+        "gflops_reference_convention": {
+            "hierarchize": hi_flops / (hier_med * 1e-3) / 1e9,
+            "evaluate": ev_flops / (eval_avg * 1e-3) / 1e9,
+            "note": "HI_FLOPS = 3 d N, EV_FLOPS = P S (11 d + 2) (adaptive_sparse_grid_bench.c:"
+                    "48-52); the walk computes far fewer flops than the convention counts",
         },
+        "roofline": roofline,
         "cpu_baseline": None,
     }
+    samples = {}
     if not args.no_forms:
-        rec.update(other_forms(g, alpha, pts, out, P, args))
+        forms, samples = other_forms(g, alpha, pts, P, args, stream)
+        rec.update(forms)
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(d, n, args.cpu_sample, args.seed)
+        baseline, reason, checks = cpu_leg(d, n, args.cpu_sample, args.seed, head, samples)
+        rec["cpu_baseline"] = baseline
+        if reason:
+            rec["cpu_baseline_unavailable"] = reason
+        elif baseline:
+            rec["speedup_vs_cpu_baseline"] = value / baseline["value"]
+        rec["checks_vs_cpu_oracle"] = checks
     print(json.dumps(rec), flush=True)
     return 0
 

@@ -91,7 +91,7 @@ int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *bas
  *   bin_min_points    evaluate launches with at least this many points bin the points
  *                     spatially first (<= 0: never; the staged variant never bins);
  *   hier_tile_values  target values per hierarchization workgroup tile (>= 64). */
-#define SG_EVAL_VARIANTS 4
+#define SG_EVAL_VARIANTS 5
 typedef struct sg_tuning {
     int eval_block;
     int eval_variant_f64;
@@ -222,6 +222,11 @@ int sg_server_submit(sg_server *server, int grid_id, const void *points, long lo
                      void *out, long long *request_id);
 int sg_server_poll(sg_server *server, long long request_id, int *done);
 int sg_server_wait(sg_server *server, long long request_id, double *latency_seconds);
+/* Fault injection for the server's error path (a test hook; it has no counterpart in
+ * the reference).  The next `launches` launch groups that hold requests of grid_id report
+ * a launch failure instead of running: those requests finish with SG_E_HIP, every other
+ * request of the same launch completes normally.  launches = 0 clears it. */
+int sg_server_inject_launch_failure(sg_server *server, int grid_id, int launches);
 int sg_server_stats(sg_server *server, long long *launches, long long *points_done,
                     long long *waiting, long long *budget_used);
 
@@ -301,6 +306,18 @@ int sg_gather_probe_f64(const sg_grid *grid, const double *d_alpha, long long nu
                         unsigned long long seed, double *d_out, void *stream);
 int sg_gather_probe_f32(const sg_grid *grid, const float *d_alpha, long long num_points,
                         unsigned long long seed, float *d_out, void *stream);
+
+/* The staged-pattern probe of the staged walk the grid runs in that precision (its tuned
+ * variant; SG_E_UNSUPPORTED if that walk is not a staged one): the walk's stage plan (the
+ * same copies into LDS and stage barriers) and, per point, one read per subspace at a
+ * random cell of its block (from LDS where the walk stages it, else from global memory)
+ * accumulated with one FMA; no hat products and no walk.  Its time is a floor for the
+ * staged walk on that grid (the walk does this work and more): the evaluate kernels'
+ * roofline.  d_out[p] = point p's sum.  No counterpart in the reference. */
+int sg_stage_probe_f64(const sg_grid *grid, const double *d_alpha, long long num_points,
+                       unsigned long long seed, double *d_out, void *stream);
+int sg_stage_probe_f32(const sg_grid *grid, const float *d_alpha, long long num_points,
+                       unsigned long long seed, float *d_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,13 @@ class RefGrid:
         self.dp = np.array([[dpp[i][s] for s in range(n)] for i in range(d)], np.int64)
         self.lidx = np.array([lip[s] for s in range(n)], np.int64)
 
+    def omp_threads(self) -> int:
+        """OpenMP threads the reference's omp_* functions run with in this process."""
+        fn = self.lib.omp_get_max_threads
+        fn.restype = C.c_int
+        fn.argtypes = []
+        return int(fn())
+
     # the reference grid values (float *sg1d, asg.c:79)
     def get_values(self) -> np.ndarray:
         return np.ctypeslib.as_array(self._sg1d_ptr, shape=(self.N,)).copy()

@@ -5,9 +5,14 @@
 //     not overlap, so the waves' copies never race on a byte;
 //   * every staged leaf of the stage finds its own surplus bytes at leaf_lds;
 //   * stages start at increasing leaves and every staged leaf lies in one stage.
+// The block-staged walk's plan (BlockStagePlan) is checked the same way against the
+// kernel's own addressing, re-derived here: the walk's static blocks in depth-first
+// order, each staged while blk_bytes fits the stage, stages filled greedily, every leaf
+// at its block's base plus blk_leaf_offset.
 // Built and run by tests/test_stage_plan.py; prints one line per grid and element size.
 #include "sg_grid.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -43,7 +48,8 @@ int check_plan(const sg::Grid& g, int e) {
         std::vector<char> written(2 * sg::kStageBytes, 0);
         for (int s = p.seg_begin[2 * k]; s < p.seg_begin[2 * k + 2]; ++s) {
             const int64_t from = p.segs[2 * s];
-            const int at = p.segs[2 * s + 1] & 0xffff, lanes = p.segs[2 * s + 1] >> 16;
+            const int at = p.segs[2 * s + 1] & sg::kSegLdsMask,
+                      lanes = p.segs[2 * s + 1] >> sg::kSegLanesShift;
             const bool is_full = s < p.seg_begin[2 * k + 1];
             if (is_full != (lanes == 64)) return fail("full segments first", g.d, elem, k);
             if (lanes < 1 || at % 16) return fail("segment shape", g.d, elem, k);
@@ -76,13 +82,123 @@ int check_plan(const sg::Grid& g, int e) {
     return 0;
 }
 
+// the static blocks of a regular grid's walk, in walk order: outer levels l_{d-1..J}
+struct BlkWalk {
+    const sg::Grid& g;
+    int J;
+    std::vector<std::vector<int>> blocks;   // outer levels (index k for dims J..d-1)
+    std::vector<int> lv;
+    void go(int k, int rem) {
+        if (k == J - 1) {
+            blocks.push_back(lv);
+            return;
+        }
+        for (int l = 0; l <= rem; ++l) {
+            lv[k] = l;
+            go(k - 1, rem - l);
+        }
+        lv[k] = 0;
+    }
+};
+
+int check_blk_plan(const sg::Grid& g, int e) {
+    const int elem = e ? 4 : 8;
+    const sg::BlockStagePlan& p = g.blk[e];
+    if (!p.ok) {
+        std::printf("d=%d n=%d elem=%d: no block plan\n", g.d, g.n, elem);
+        return 0;
+    }
+    const int J = std::min(g.d, sg::kStageJ), nm1 = g.n - 1, STB = sg::blk_stage_bytes(elem);
+    const int K = p.num_stages;
+    if (int(p.seg_begin.size()) != 2 * K + 1) return fail("blk seg_begin has 2K+1 entries", g.d, elem, -1);
+    if (int(p.segs.size()) != 2 * (p.seg_begin[2 * K] + 4))
+        return fail("blk descriptor table ends in four empty entries", g.d, elem, -1);
+    std::vector<uint8_t> src(size_t(g.N) * elem);
+    for (size_t i = 0; i < src.size(); ++i) src[i] = uint8_t((i * 2654435761u) >> 11);
+    BlkWalk w{g, J, {}, std::vector<int>(g.d, 0)};
+    w.go(g.d - 1, nm1);
+    // the kernel's stage assignment
+    struct Leaf { int64_t base, bytes; int lds; };
+    std::vector<std::vector<Leaf>> stage_leaves;
+    int used = 0, buf = -1;
+    int64_t staged = 0, total = 0;
+    for (const auto& outer : w.blocks) {
+        int H = 0;
+        for (int k = J; k < g.d; ++k) H += outer[k];
+        const int R = nm1 - H;
+        const int64_t C = sg::stage_binom(R + J, J);
+        total += C;
+        const int64_t bytes = sg::blk_bytes(J, R, H, elem);
+        if (bytes > STB) continue;
+        if (buf < 0 || used + bytes > STB) {
+            buf = buf < 0 ? 0 : buf ^ 1;
+            used = 0;
+            stage_leaves.emplace_back();
+        }
+        // leaves of the block: inner levels with sum <= R
+        std::vector<int> l(outer);
+        std::vector<int> v(J, 0);
+        for (;;) {
+            int s = 0;
+            for (int j = 0; j < J; ++j) { l[j] = v[j]; s += v[j]; }
+            if (s <= R) {
+                const int64_t id = sg::subspace_id(g, l.data());
+                if (id < 0) return fail("blk leaf not in the grid", g.d, elem, -1);
+                int inner[sg::kMaxDims] = {};
+                for (int j = 0; j < J; ++j) inner[j] = v[j];
+                const int lds = buf * STB + used + int(sg::blk_leaf_offset(J, inner, H, elem));
+                stage_leaves.back().push_back({g.sub_base[id], (int64_t(1) << (H + s)) * elem, lds});
+                ++staged;
+            }
+            int j = 0;
+            while (j < J && ++v[j] > R) v[j++] = 0;
+            if (j == J) break;
+        }
+        used += int(bytes);
+    }
+    if (total != g.S) return fail("blk walk does not cover the grid", g.d, elem, -1);
+    if (int(stage_leaves.size()) != K) return fail("blk stage count", g.d, elem, -1);
+    int64_t full = 0, partial = 0;
+    for (int k = 0; k < K; ++k) {
+        std::vector<uint8_t> lds(2 * STB, 0);
+        std::vector<char> written(2 * STB, 0);
+        for (int s = p.seg_begin[2 * k]; s < p.seg_begin[2 * k + 2]; ++s) {
+            const int64_t from = p.segs[2 * s];
+            const int at = p.segs[2 * s + 1] & sg::kSegLdsMask,
+                      lanes = p.segs[2 * s + 1] >> sg::kSegLanesShift;
+            const bool is_full = s < p.seg_begin[2 * k + 1];
+            if (is_full != (lanes == 64)) return fail("blk full segments first", g.d, elem, k);
+            if (lanes < 1 || at % 16) return fail("blk segment shape", g.d, elem, k);
+            if (at / STB != k % 2 || (at + 16 * lanes - 1) / STB != k % 2)
+                return fail("blk segment outside the stage's buffer", g.d, elem, k);
+            for (int b = 0; b < 16 * lanes; ++b) {
+                if (written[at + b]) return fail("blk segments overlap", g.d, elem, k);
+                written[at + b] = 1;
+                const int64_t o = from + b;
+                lds[at + b] = o < int64_t(src.size()) ? src[size_t(o)] : 0;
+            }
+            (is_full ? full : partial) += 1;
+        }
+        for (const Leaf& lf : stage_leaves[k]) {
+            if (lf.lds % elem) return fail("blk leaf alignment", g.d, elem, k);
+            for (int64_t b = 0; b < lf.bytes; ++b)
+                if (!written[lf.lds + b] || lds[lf.lds + b] != src[size_t(lf.base * elem + b)])
+                    return fail("blk leaf bytes differ from its surpluses", g.d, elem, k);
+        }
+    }
+    std::printf("d=%d n=%d elem=%d block-staged: %d stages, %lld full + %lld partial segments, %lld of %lld leaves staged: ok\n",
+                g.d, g.n, elem, K, (long long)full, (long long)partial, (long long)staged,
+                (long long)g.S);
+    return 0;
+}
+
 int check_regular(int d, int n, const int* limits) {
     sg::Grid g;
     if (sg::build_regular(g, d, n, limits) != 0 || sg::build_launch_tables(g, 512) != 0) {
         std::printf("FAIL d=%d n=%d: grid\n", d, n);
         return 1;
     }
-    return check_plan(g, 0) | check_plan(g, 1);
+    return check_plan(g, 0) | check_plan(g, 1) | check_blk_plan(g, 0) | check_blk_plan(g, 1);
 }
 
 // the regular level set of dimension d and level sums below n, plus l_0 up to top
@@ -119,6 +235,8 @@ int main() {
     r |= check_regular(1, 8, nullptr);
     r |= check_regular(2, 8, nullptr);
     r |= check_regular(3, 5, nullptr);
+    r |= check_regular(5, 9, nullptr);
+    r |= check_regular(16, 4, nullptr);
     const int lim6[6] = {3, 1, 3, 2, 6, 6};
     r |= check_regular(6, 7, lim6);
     const int lim3[3] = {1, 5, 5};

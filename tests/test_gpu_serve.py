@@ -117,3 +117,57 @@ def test_budget_admission_and_errors(sg, torch_gpu, resident):
         assert np.array_equal(out, alone(torch, g, a, pts))
     assert srv.stats()["budget_used"] == 0
     srv.close()
+
+
+def test_injected_launch_failure_reaches_only_its_requests(sg, torch_gpu, resident):
+    """A launch group that fails (injected on the host: the kernels never run, so nothing
+    faults) fails its own requests with SG_E_HIP; requests of the other grids batched in
+    the same launch, and later requests of the failing grid, complete bit-identical to
+    their alone results."""
+    torch = torch_gpu
+    srv = sg.Server("f64", slots=1, max_batch_points=1 << 14, max_requests=64)
+    ids = [srv.add_grid(g, a) for g, a in resident]
+    # grids 0 (d=3) and 1 (d=10) run in different launch groups of one launch
+    srv.inject_launch_failure(ids[1], 1)
+    reqs = []
+    for k in range(12):
+        gi = k % 2
+        pts = W.uniform_points(100 + 7 * k, resident[gi][0].d, seed=700 + k)
+        out = np.full(len(pts), np.nan)
+        reqs.append((gi, pts, out, srv.submit(ids[gi], pts, out)))
+    failed, ok = [], []
+    for gi, pts, out, rid in reqs:
+        try:
+            srv.wait(rid)
+            ok.append((gi, pts, out))
+        except sg.SgError as e:
+            assert e.status == sg.SG_E_HIP and "injected" in str(e)
+            failed.append(gi)
+    assert failed and set(failed) == {1}, failed
+    assert any(gi == 0 for gi, _, _ in ok)
+    for gi, pts, out in ok:
+        g, a = resident[gi]
+        assert np.array_equal(out, alone(torch, g, a, pts))
+    # the injection was used up: the failing grid serves again
+    pts = W.uniform_points(300, resident[1][0].d, seed=799)
+    out = np.full(300, np.nan)
+    srv.wait(srv.submit(ids[1], pts, out))
+    assert np.array_equal(out, alone(torch, resident[1][0], resident[1][1], pts))
+    assert srv.stats()["budget_used"] == 0
+    srv.close()
+
+
+def test_server_rejects_oversized_grid(sg, torch_gpu):
+    """A grid whose surpluses exceed the evaluate kernels' 2 GiB buffer range in the
+    server's precision is refused at add_grid (FP32 admits it)."""
+    torch = torch_gpu
+    big = sg.Grid(4, 21, [7, 7, 7, 7])       # N = 419,822,593: FP64 bytes > 2^31 > FP32 bytes
+    dummy = torch.zeros(1, dtype=torch.float64, device="cuda")   # never read
+    srv = sg.Server("f64", slots=1, max_batch_points=1 << 12, max_requests=4)
+    with pytest.raises(sg.SgError) as e:
+        srv.add_grid(big, dummy)
+    assert e.value.status == sg.SG_E_RANGE
+    srv.close()
+    srv32 = sg.Server("f32c", slots=1, max_batch_points=1 << 12, max_requests=4)
+    srv32.add_grid(big, dummy.float())
+    srv32.close()

@@ -1,8 +1,10 @@
-"""The staged walk (evaluate variant 3): the workgroup copies each coming subspace block
-into LDS with buffer-to-LDS loads and every lane reads its coefficient there.
+"""The staged walks (evaluate variants 3 and 4): the workgroup copies each coming subspace
+block into LDS with buffer-to-LDS loads and every lane reads its coefficient there;
+variant 4 lays the blocks out in walk order with compile-time offsets (regular grids with
+n - 1 in 2..7 and the dimensions it is built for; other grids run variant 3).
 
-Bars: the staged walk visits the same leaves in the same order and does the same
-arithmetic per point as variant 0 (three tabled dimensions, batches of 8), so its
+Bars: the staged walks visit the same leaves in the same order and do the same
+arithmetic per point as variant 0 (three tabled dimensions, batches of 8), so their
 outputs are bit-identical to variant 0's in FP64 and FP32-Kahan; and they satisfy the
 FP64 bound against the oracle, |gpu - oracle| <= 1e-12 * sum_j |term_j|.  Shapes cover
 one and many stages, blocks too large for a stage (gathered inside the staged walk),
@@ -19,6 +21,7 @@ pytestmark = pytest.mark.gpu
 
 REL_F64 = 1e-12
 STAGED = 3
+BLK = 4
 
 
 @pytest.fixture(scope="module")
@@ -45,7 +48,7 @@ def check_f64(out, ref, absref):
 
 def run_variants(torch, g, alpha, pts, dtype, binning):
     outs = {}
-    for v in (0, STAGED):
+    for v in (0, STAGED, BLK):
         g.set_tuning(eval_variant_f64=v, eval_variant_f32=v, bin_min_points=binning)
         o = torch.full((len(pts),), float("nan"), dtype=dtype, device="cuda")
         g.evaluate(dev(torch, alpha).to(dtype), dev(torch, pts).to(dtype), o)
@@ -78,9 +81,11 @@ def test_staged_equals_variant0_and_oracle(sg, torch_gpu, d, n, lim, P, binning)
     g = sg.Grid(d, n, lim)
     o64 = run_variants(torch, g, alpha, pts, torch.float64, binning)
     assert np.array_equal(o64[STAGED], o64[0])
+    assert np.array_equal(o64[BLK], o64[0])
     check_f64(o64[STAGED], ref, absref)
     o32 = run_variants(torch, g, alpha, pts, torch.float32, binning)
     assert np.array_equal(o32[STAGED], o32[0])
+    assert np.array_equal(o32[BLK], o32[0])
     err = np.abs(o32[STAGED].astype(np.float64) - ref)
     assert (err <= 2e-6 * absref + 1e-7).all(), err.max()
 
@@ -116,9 +121,11 @@ def test_staged_explicit_sets(sg, torch_gpu):
         for binning in (-1, 1):
             o64 = run_variants(torch, g, alpha, pts, torch.float64, binning)
             assert np.array_equal(o64[STAGED], o64[0]), (seed, binning)
+            assert np.array_equal(o64[BLK], o64[0]), (seed, binning)
             check_f64(o64[STAGED], ref, absref)
             o32 = run_variants(torch, g, alpha, pts, torch.float32, binning)
             assert np.array_equal(o32[STAGED], o32[0]), (seed, binning)
+            assert np.array_equal(o32[BLK], o32[0]), (seed, binning)
 
 
 def test_staged_batch_and_graph(sg, torch_gpu):

@@ -467,6 +467,69 @@ __device__ __forceinline__ void block_batches_blk(const Ln& ln, T pp, uint32_t v
     }
 }
 
+// A block too large for a stage is staged by slices: slice a holds the leaves with
+// l_{J-1} = a, laid out as a block over the J-1 inner tabled dimensions below outer level
+// sum H + a, and every slice takes its own place in the stages (possibly the next
+// stage).  The leaves, their order and the block's arithmetic (nested sums carried from
+// slice to slice) are those of the whole block; only where the reads come from changes.
+template <int J, int R>
+__host__ __device__ constexpr int slice_first(int a) {     // first leaf with l_{J-1} = a
+    int c = 0;
+    for (int b = 0; b < a; ++b) c += binom(R - b + J - 1, J - 1);
+    return c;
+}
+
+template <int J, int R, int c>
+__host__ __device__ constexpr int64_t slice_leaf_lds(int H, int elem) {
+    int lv[J] = {};
+    for (int j = 0; j < J - 1; ++j) lv[j] = Simplex<J, R>().l[c][j];
+    return blk_leaf_offset(J - 1, lv, H + Simplex<J, R>().l[c][J - 1], elem);
+}
+
+template <int J, int R, int H, int B, int c, int cend, class Ln, class T, class St>
+__device__ __forceinline__ void slice_issue(T (&a)[B], const Ln& ln, uint32_t vb, St& st) {
+    if constexpr (c < cend) {
+        constexpr uint32_t lo = uint32_t(slice_leaf_lds<J, R, c>(H, int(sizeof(T))));
+        a[c % B] = st.fetch_lds_at(leaf_offset_ct<J, R, c, J - 1, H>(ln, vb) + lo);
+        slice_issue<J, R, H, B, c + 1, cend>(a, ln, vb, st);
+    }
+}
+
+template <int J, int R, int H, int B, int c0, int c1, class Ln, class T, class St>
+__device__ __forceinline__ void slice_batches(const Ln& ln, T pp, uint32_t vb, St& st,
+                                              T (&sn)[J]) {
+    if constexpr (c0 < c1) {
+        constexpr int cend = (c0 + B < c1) ? c0 + B : c1;
+        T a[B];
+        slice_issue<J, R, H, B, c0, cend>(a, ln, vb, st);
+        if constexpr (kNestedSum<decltype(st.acc)>)
+            block_consume_nested<J, R, B, c0, cend>(a, sn, ln, pp, st.acc);
+        else
+            block_consume<J, R, B, c0, cend>(a, ln, pp, st.acc);
+        // a slice is one long straight-line stretch: keep each batch's reads with it (the
+        // scheduler otherwise hoists the whole slice's reads and the registers spill)
+        __builtin_amdgcn_sched_barrier(0);
+        slice_batches<J, R, H, B, cend, c1>(ln, pp, vb, st, sn);
+    }
+}
+
+template <int J, int R, int H, int B, int a, class Ln, class T, class St>
+__device__ __forceinline__ void block_slices(const Ln& ln, T pp, uint32_t off, St& st,
+                                             T (&sn)[J]) {
+    if constexpr (a <= R) {
+        // the nested sums of the slices so far must exist here, before the stage check: the
+        // compiler could otherwise sink their FMAs past it to the block's end and keep all
+        // the slices' loaded values alive
+#pragma unroll
+        for (int j = 0; j < J; ++j) asm volatile("" : "+v"(sn[j]));
+        st.template block_begin<uint32_t(blk_bytes(J - 1, R - a, H + a, int(sizeof(T))))>();
+        launder_offsets<J, R>(const_cast<Ln&>(ln));
+        slice_batches<J, R, H, B, slice_first<J, R>(a), slice_first<J, R>(a + 1)>(
+            ln, pp, off + st.base, st, sn);
+        block_slices<J, R, H, B, a + 1>(ln, pp, off, st, sn);
+    }
+}
+
 // All C(R+J, J) leaves below a depth-J node whose remaining level budget R fits inside
 // the J tabled dimensions.  The staged walk reads a block from LDS when the host packed
 // it into a stage (the same rule, stage_whole, decides on both sides), else from global
@@ -477,10 +540,13 @@ __device__ __forceinline__ void static_block(const Ln& ln, T pp, uint32_t off, i
     T sn[J];                   // nested partial sums (FP64 blocks)
     if constexpr (St::kBlk) {
         constexpr int H = St::kNM1 - R;
-        if constexpr (H >= 0 && blk_bytes(J, R, H, int(sizeof(T))) <= St::kStageBytes) {
+        constexpr int kHow = blk_staging(J, R, H, int(sizeof(T)), St::kStageBytes);
+        if constexpr (kHow == 1) {
             st.template block_begin<uint32_t(blk_bytes(J, R, H, int(sizeof(T))))>();
             launder_offsets<J, R>(const_cast<Ln&>(ln));   // ln is the kernel's own lane
             block_batches_blk<J, R, H, B, 0>(ln, pp, off + st.base, st, sn);
+        } else if constexpr (kHow == 2) {
+            block_slices<J, R, H, B, 0>(ln, pp, off, st, sn);
         } else {
             block_batches<J, R, B, false, 0>(ln, pp, off, hsum, st, sn);
         }
@@ -639,6 +705,37 @@ template <class T>
 constexpr bool blk_lean_walk() {
     return sizeof(T) == 8;
 }
+// Dimension J's levels 0..rem above the static blocks, as one switch that falls through:
+// entering at case rem runs the static blocks of budgets rem, rem - 1, ..., 0 (level
+// l = rem - R of dimension J each), so a node of dimension J+1 dispatches once instead
+// of once per static block, and every static block's code exists once.  Used up to
+// d = 10 (d = 10, n = 8: 146.6 -> 144.1 ms); from d = 12 on it is neutral to 15 % slower.
+#ifndef SG_BLK_FALL_DMAX
+#define SG_BLK_FALL_DMAX 10
+#endif
+template <int J, int B, int R, class Ln, class T, class St>
+__device__ __forceinline__ void blk_level_run(Ln& ln, T pp, uint32_t off, int hsum, int rem,
+                                              St& st) {
+    T phi;
+    uint32_t t;
+    const int l = rem - R;
+    hat_at(ln.x[J], l, phi, t);
+    const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
+    static_block<J, R, B>(ln, pp * phi, o, hsum + l, st);
+}
+
+template <int J, int B, class Ln, class T, class St>
+__device__ __forceinline__ void blk_levels(Ln& ln, T pp, uint32_t off, int hsum, int rem,
+                                           St& st) {
+    switch (rem) {
+#define SG_LV(RR) \
+    case RR: blk_level_run<J, B, RR>(ln, pp, off, hsum, rem, st); [[fallthrough]];
+        SG_LV(7) SG_LV(6) SG_LV(5) SG_LV(4) SG_LV(3) SG_LV(2) SG_LV(1) SG_LV(0)
+#undef SG_LV
+        default: break;
+    }
+}
+
 template <int K, int D, int J, int B, class T, class St>
 struct WalkerBlk {
     using Ln = LaneBlk<D, J, T>;
@@ -646,6 +743,8 @@ struct WalkerBlk {
                                                St& st) {
         if constexpr (K == J - 1) {
             static_block_dispatch<J, 8, B>(rem, ln, pp, off, hsum, st);
+        } else if constexpr (K == J && D <= SG_BLK_FALL_DMAX) {
+            blk_levels<J, B>(ln, pp, off, hsum, rem, st);
         } else {
             for (int l = 0; l <= rem; ++l) {
                 T phi;
@@ -828,23 +927,22 @@ void k_eval(Params<T> prm) {
 }
 
 // The staged-pattern probe: the data path of a staged walk without its arithmetic.  The
-// workgroup runs the walk's stage plan (the same buffer-to-LDS copies and stage
-// barriers); each lane visits the S leaves in walk order and reads one coefficient per
-// leaf at a random cell of the leaf's block, from LDS for staged leaves and by a buffer
-// load for the others, and accumulates it with one FMA.  No hats, no walk.  Its time is a
-// floor for the staged walk on the same grid: the walk does all of this and more.
+// workgroup runs the walk's stage plan (the same buffer-to-LDS copies and stage barriers
+// between them); in stage k every lane reads as many coefficients from LDS as the walk
+// reads there (the stage's staged leaves), at random cells of the stage, and adds each
+// with one FMA; then it reads the leaves the walk gathers from global memory, one random
+// cell of each leaf's block.  No hats, no walk, no per-leaf addressing: it does strictly
+// less work per read than the walk, so its time is a floor for the walk's on the grid.
 template <class T>
 struct ProbeParams {
     const T* alpha;
     int64_t N;
-    const int32_t* leaf_lds;     // [S] LDS byte offset (-1: read from global memory)
-    const int32_t* leaf_base;    // [S] base offsets in walk order
-    const uint8_t* leaf_lsum;    // [S] level sums in walk order
-    const int32_t* stage_first;  // [K+1]
+    const int32_t* info;         // [K][2]: staged leaves of stage k, cell mask of the stage
+    const int32_t* glob;         // [G][2]: base offset and level sum of each gathered leaf
     const int32_t* seg_begin;
     const int32_t* segs;
     int num_stages;
-    int S;
+    int num_glob;
     int64_t P;
     uint32_t seed;
     T* out;
@@ -856,10 +954,8 @@ __global__ __launch_bounds__(TPB) void k_stage_probe(ProbeParams<T> p) {
     lds_byte* lds = (lds_byte*)buf;
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<T*>(p.alpha), short(0), int(p.N * int64_t(sizeof(T))), 0x00020000);
-    const cptr<int32_t> leaf_lds = as_const(p.leaf_lds), leaf_base = as_const(p.leaf_base),
-                        stage_first = as_const(p.stage_first), seg_begin = as_const(p.seg_begin),
-                        segs = as_const(p.segs);
-    const cptr<uint8_t> lsum = as_const(p.leaf_lsum);
+    const cptr<int32_t> info = as_const(p.info), glob = as_const(p.glob),
+                        seg_begin = as_const(p.seg_begin), segs = as_const(p.segs);
     const int64_t pi = int64_t(blockIdx.x) * TPB + threadIdx.x;
     uint32_t h = uint32_t(pi) * 0x9E3779B9u ^ p.seed;
     h ^= h >> 15;
@@ -868,40 +964,42 @@ __global__ __launch_bounds__(TPB) void k_stage_probe(ProbeParams<T> p) {
     const T w = T(1) + T(h & 1023u) * T(1.0 / 4096);
     T acc = T(0);
     const int K = p.num_stages;
-    int stage = 0, next = K > 0 ? stage_first[0] : p.S;
     if (K > 0) {
         copy_stage<TPB>(rsrc, lds, seg_begin, segs, 0);
         stage_barrier();
         if (K > 1) copy_stage<TPB>(rsrc, lds, seg_begin, segs, 1);
-        next = K > 1 ? stage_first[1] : p.S;
     }
-    auto one = [&](int q) -> T {
-        const uint32_t cell = (h >> (q & 7)) & ((1u << lsum[q]) - 1u);
-        const int32_t at = leaf_lds[q];
-        if (at >= 0)
-            return *reinterpret_cast<const __attribute__((address_space(3))) T*>(
-                lds + (uint32_t(at) + cell * uint32_t(sizeof(T))));
-        return gather_leaf<T>(rsrc, leaf_base[q], cell * uint32_t(sizeof(T)));
-    };
-    int q = 0;
-    while (q < p.S) {
-        if (q == next) {                       // the walk reaches stage + 1
+    for (int k = 0; k < K; ++k) {
+        if (k > 0) {
             stage_barrier();
-            ++stage;
-            next = stage + 1 < K ? stage_first[stage + 1] : p.S;
-            if (stage + 1 < K) copy_stage<TPB>(rsrc, lds, seg_begin, segs, stage + 1);
+            if (k + 1 < K) copy_stage<TPB>(rsrc, lds, seg_begin, segs, k + 1);
         }
-        const int qe = min(min(q + 8, next), p.S);
-        if (qe - q == 8) {
-            T a[8];
+        const int cnt = info[2 * k];
+        const uint32_t mask = uint32_t(info[2 * k + 1]);
+        const uint32_t base = uint32_t(k & 1) * uint32_t(STB);
+        uint32_t at[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) a[i] = one(q + i);
+        for (int j = 0; j < 8; ++j)
+            at[j] = base + (((h >> j) ^ (h >> (j + 11))) & mask) * uint32_t(sizeof(T));
+        int i = 0;
+        for (; i + 8 <= cnt; i += 8) {
+            asm volatile("" ::: "memory");     // re-read every pass (no load hoisting)
+            T v[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) acc = fma(a[i], w, acc);
-        } else {
-            for (int i = q; i < qe; ++i) acc = fma(one(i), w, acc);
+            for (int j = 0; j < 8; ++j)
+                v[j] = *reinterpret_cast<const __attribute__((address_space(3))) T*>(lds + at[j]);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc = fma(v[j], w, acc);
         }
-        q = qe;
+        for (; i < cnt; ++i) {
+            asm volatile("" ::: "memory");
+            acc = fma(*reinterpret_cast<const __attribute__((address_space(3))) T*>(lds + at[i & 7]),
+                      w, acc);
+        }
+    }
+    for (int q = 0; q < p.num_glob; ++q) {
+        const uint32_t cell = (h >> (q & 7)) & ((1u << glob[2 * q + 1]) - 1u);
+        acc = fma(gather_leaf<T>(rsrc, glob[2 * q], cell * uint32_t(sizeof(T))), w, acc);
     }
     if (pi < p.P) p.out[pi] = acc;
 }
@@ -1104,36 +1202,30 @@ int launch_stage_probe(const Grid& g, const DeviceTables& t, int variant, const 
         return kUnsupported;
     }
     const GridDesc& hd = t.host_desc;
+    const DeviceTables::Probe& pt = t.probe[v == kBlkVariant ? 1 : 0][e];
     ProbeParams<T> p{};
     p.alpha = alpha;
     p.N = g.N;
-    p.leaf_base = hd.leaf_base;
-    p.leaf_lsum = hd.leaf_lsum;
-    p.S = int(g.S);
+    p.info = pt.info;
+    p.glob = pt.glob;
+    p.num_stages = pt.num_stages;
+    p.num_glob = pt.num_glob;
     p.P = P;
     p.seed = seed;
     p.out = out;
-    hipError_t err;
     if (v == kBlkVariant) {
-        p.leaf_lds = hd.blk[e].leaf_lds;
-        p.stage_first = hd.blk[e].stage_first;
         p.seg_begin = hd.blk[e].seg_begin;
         p.segs = hd.blk[e].segs;
-        p.num_stages = hd.blk[e].num_stages;
         constexpr int TPB = blk_threads(int(sizeof(T)));
         hipLaunchKernelGGL((k_stage_probe<T, TPB, blk_stage_bytes(int(sizeof(T)))>),
                            dim3(unsigned((P + TPB - 1) / TPB)), dim3(TPB), 0, s, p);
-        err = hipGetLastError();
     } else {
-        p.leaf_lds = hd.stage[e].leaf_lds;
-        p.stage_first = hd.stage[e].stage_first;
         p.seg_begin = hd.stage[e].seg_begin;
         p.segs = hd.stage[e].segs;
-        p.num_stages = hd.stage[e].num_stages;
         hipLaunchKernelGGL((k_stage_probe<T, kThreads, kStageBytes>),
                            dim3(unsigned((P + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, p);
-        err = hipGetLastError();
     }
+    const hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_error(std::string("stage probe launch: ") + hipGetErrorString(err));
         return kHip;

@@ -532,44 +532,46 @@ void build_blk_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 
         const auto it = std::lower_bound(g.sub_base.begin(), g.sub_base.end(), base);
         return (it == g.sub_base.end() || *it != base) ? -1 : int64_t(it - g.sub_base.begin());
     };
-    for (const auto& blk : blocks) {
-        const int32_t q0 = blk[0], R = blk[1], H = blk[2];
-        if (H != nm1 - R) return;                       // not the regular-walk invariant
-        const int64_t bytes = blk_bytes(J, R, H, elem);
-        if (bytes > stage_bytes) continue;              // gathered from global memory
+    // place `bytes` of the next staged unit (a whole block or one slice) in the stages
+    auto place = [&](int64_t bytes, int32_t q_first) -> int32_t {
         if (buf < 0 || used + bytes > stage_bytes) {
             close_stage();
             buf = buf < 0 ? 0 : buf ^ 1;
             used = 0;
-            p.stage_first.push_back(q0);
+            p.stage_first.push_back(q_first);
         }
         const int32_t at = buf * stage_bytes + used;
-        // leaves of run s: (HBM base, LDS offset by the formula)
-        std::vector<std::vector<std::pair<int32_t, int64_t>>> runs(size_t(R) + 1);
-        const int64_t C = stage_binom(R + J, J);
+        used += int(bytes);
+        return at;
+    };
+    // lay out leaves [c0, c1) of the block at q0 as a block over the Jb inner dimensions
+    // below outer sum Hb (Jb = J: the whole block; Jb = J - 1: one slice) at LDS `at`
+    auto lay_out = [&](int32_t q0, int c0, int c1, int Jb, int Hb, int32_t at) -> bool {
+        std::vector<std::vector<std::pair<int32_t, int64_t>>> runs(size_t(nm1) + 1);
         int lv[kMaxDims] = {};
-        for (int64_t c = 0; c < C; ++c) {
+        for (int c = c0; c < c1; ++c) {
             const int32_t base = g.dfs_base[size_t(q0 + c)];
             const int64_t id = subspace_of(base);
-            if (id < 0) return;
+            if (id < 0) return false;
             int s = 0;
-            for (int j = 0; j < J; ++j) {
+            for (int j = 0; j < Jb; ++j) {
                 lv[j] = g.sub_levels[size_t(id * g.d + j)];
                 s += lv[j];
             }
-            if (s > R) return;
-            runs[size_t(s)].emplace_back(base, blk_leaf_offset(J, lv, H, elem));
-            p.leaf_lds[size_t(q0 + c)] = at + int32_t(blk_leaf_offset(J, lv, H, elem));
+            const int64_t lo = blk_leaf_offset(Jb, lv, Hb, elem);
+            runs[size_t(s)].emplace_back(base, lo);
+            p.leaf_lds[size_t(q0 + c)] = at + int32_t(lo);
         }
-        for (int s = 0; s <= R; ++s) {
+        for (int s = 0; s <= nm1; ++s) {
             auto& rl = runs[size_t(s)];
+            if (rl.empty()) continue;
             std::sort(rl.begin(), rl.end());
-            const int64_t sub_elems = int64_t(1) << (H + s);
-            const int64_t run_off = blk_run_offset(J, s, H, elem);
-            if (int64_t(rl.size()) != stage_binom(s + J - 1, J - 1)) return;
+            const int64_t sub_elems = int64_t(1) << (Hb + s);
+            const int64_t run_off = blk_run_offset(Jb, s, Hb, elem);
+            if (int64_t(rl.size()) != stage_binom(s + Jb - 1, Jb - 1)) return false;
             for (size_t i = 0; i < rl.size(); ++i) {
-                if (rl[i].first != rl[0].first + int64_t(i) * sub_elems) return;   // contiguous
-                if (rl[i].second != run_off + int64_t(i) * sub_elems * elem) return;
+                if (rl[i].first != rl[0].first + int64_t(i) * sub_elems) return false;   // contiguous
+                if (rl[i].second != run_off + int64_t(i) * sub_elems * elem) return false;
             }
             const int32_t len = int32_t(int64_t(rl.size()) * sub_elems * elem);
             const int64_t src = int64_t(rl[0].first) * elem;
@@ -580,7 +582,25 @@ void build_blk_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 
                 to.push_back((at + int32_t(run_off) + o) | (lanes << kSegLanesShift));
             }
         }
-        used += int(bytes);
+        return true;
+    };
+    for (const auto& blk : blocks) {
+        const int32_t q0 = blk[0], R = blk[1], H = blk[2];
+        if (H != nm1 - R) return;                       // not the regular-walk invariant
+        const int how = blk_staging(J, R, H, elem, stage_bytes);
+        if (how == 1) {
+            const int32_t at = place(blk_bytes(J, R, H, elem), q0);
+            if (!lay_out(q0, 0, int(stage_binom(R + J, J)), J, H, at)) return;
+        } else if (how == 2) {
+            // slices a = 0..R: leaves with l_{J-1} = a, consecutive in walk order
+            int c = 0;
+            for (int a = 0; a <= R; ++a) {
+                const int cnt = int(stage_binom(R - a + J - 1, J - 1));
+                const int32_t at = place(blk_bytes(J - 1, R - a, H + a, elem), q0 + c);
+                if (!lay_out(q0, c, c + cnt, J - 1, H + a, at)) return;
+                c += cnt;
+            }
+        }
     }
     close_stage();
     p.stage_first.push_back(int32_t(g.S));

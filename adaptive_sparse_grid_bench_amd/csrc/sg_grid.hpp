@@ -108,7 +108,7 @@ SG_HOST_DEVICE constexpr int blk_stage_bytes(int elem) {
 constexpr int kBlkMinNM1 = 2;
 constexpr int kBlkMaxNM1 = 7;
 SG_HOST_DEVICE constexpr bool blk_built(int d, int elem) {
-    return elem == 8 ? (d >= 11 && d <= 20) : (d >= 7 && d <= 16);
+    return elem == 8 ? (d >= 9 && d <= 20) : (d >= 7 && d <= 16);
 }
 
 // bytes of run s (level sum s inside the block) of a static block over J tabled
@@ -141,6 +141,19 @@ SG_HOST_DEVICE constexpr int64_t blk_leaf_offset(int J, const int* lv, int H, in
     int s = 0;
     for (int j = 0; j < J; ++j) s += lv[j];
     return blk_run_offset(J, s, H, elem) + blk_run_pos(J, lv) * (int64_t(elem) << (H + s));
+}
+
+// The block of budget R below outer level sum H fits a stage whole (1), by slices (2:
+// slice a = the leaves with l_{J-1} = a, a block over the J-1 inner tabled dimensions
+// below outer level sum H + a), or is gathered from global memory (0).  The kernel and
+// the host plan apply the same rule.
+SG_HOST_DEVICE constexpr int blk_staging(int J, int R, int H, int elem, int stage_bytes) {
+    if (H < 0) return 0;
+    if (blk_bytes(J, R, H, elem) <= stage_bytes) return 1;
+    if (J < 2) return 0;
+    for (int a = 0; a <= R; ++a)
+        if (blk_bytes(J - 1, R - a, H + a, elem) > stage_bytes) return 0;
+    return 2;
 }
 
 // How the block-staged walk copies one grid's blocks into LDS, for one element size.

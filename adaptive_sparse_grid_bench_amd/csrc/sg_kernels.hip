@@ -859,20 +859,55 @@ int upload_tables(const Grid& g, DeviceTables& t) {
     if ((st = upload(t.dfs_base, g.dfs_base.data(), g.dfs_base.size())) != kOk) return st;
     if ((st = upload(t.dfs_bound, g.dfs_bound.data(), g.dfs_bound.size())) != kOk) return st;
     {
-        std::vector<uint8_t> ls(size_t(g.S), 0);
+        // staged-pattern probe tables from the walk's stage plans
+        std::vector<int32_t> lsum(size_t(g.S), 0);
         for (int64_t q = 0; q < g.S; ++q) {
             const auto it = std::lower_bound(g.sub_base.begin(), g.sub_base.end(), g.dfs_base[q]);
             const int64_t id = it - g.sub_base.begin();
-            int s = 0;
-            for (int j = 0; j < g.d; ++j) s += g.sub_levels[size_t(id * g.d + j)];
-            ls[size_t(q)] = uint8_t(s);
+            int sum = 0;
+            for (int j = 0; j < g.d; ++j) sum += g.sub_levels[size_t(id * g.d + j)];
+            lsum[size_t(q)] = sum;
         }
-        if ((st = upload(t.walk_lsum, ls.data(), ls.size())) != kOk) return st;
+        for (int kind = 0; kind < 2; ++kind)
+            for (int e = 0; e < 2; ++e) {
+                const bool ok = kind == 0 ? g.stage[e].ok : g.blk[e].ok;
+                if (!ok) continue;
+                const std::vector<int32_t>& lds = kind == 0 ? g.stage[e].leaf_lds : g.blk[e].leaf_lds;
+                const std::vector<int32_t>& first =
+                    kind == 0 ? g.stage[e].stage_first : g.blk[e].stage_first;
+                const int elem = e == 0 ? 8 : 4;
+                const int K = int(first.size()) - 1;
+                std::vector<int32_t> info, glob;
+                for (int k = 0; k < K; ++k) {
+                    int32_t cnt = 0, lo = INT32_MAX, hi = 0;
+                    for (int32_t q = first[k]; q < first[k + 1]; ++q) {
+                        if (lds[size_t(q)] < 0) continue;
+                        ++cnt;
+                        lo = std::min(lo, lds[size_t(q)]);
+                        hi = std::max(hi, lds[size_t(q)] + (int32_t(1) << lsum[size_t(q)]) * elem);
+                    }
+                    int64_t cells = cnt ? (hi - lo) / elem : 1, pw = 1;
+                    while (pw * 2 <= cells) pw *= 2;     // largest power of two inside the stage
+                    info.push_back(cnt);
+                    info.push_back(int32_t(pw - 1));
+                }
+                for (int64_t q = 0; q < g.S; ++q)
+                    if (lds[size_t(q)] < 0) {
+                        glob.push_back(g.dfs_base[size_t(q)]);
+                        glob.push_back(lsum[size_t(q)]);
+                    }
+                DeviceTables::Probe& pr = t.probe[kind][e];
+                pr.num_stages = K;
+                pr.num_glob = int(glob.size() / 2);
+                if (info.empty()) info.push_back(0);
+                if (glob.empty()) glob.push_back(0);
+                if ((st = upload(pr.info, info.data(), info.size())) != kOk) return st;
+                if ((st = upload(pr.glob, glob.data(), glob.size())) != kOk) return st;
+            }
     }
     eval::GridDesc desc{};
     desc.leaf_base = t.dfs_base;
     desc.bounds = t.dfs_bound;
-    desc.leaf_lsum = t.walk_lsum;
     desc.N = g.N;
     desc.n = g.n;
     for (int j = 0; j < g.d; ++j) desc.limits[j] = g.limits[j];
@@ -899,14 +934,10 @@ int upload_tables(const Grid& g, DeviceTables& t) {
         if (!p.ok) continue;
         std::vector<int32_t> all(p.seg_begin);
         all.insert(all.end(), p.segs.begin(), p.segs.end());
-        all.insert(all.end(), p.leaf_lds.begin(), p.leaf_lds.end());
-        all.insert(all.end(), p.stage_first.begin(), p.stage_first.end());
         if ((st = upload(t.blk_tabs[e], all.data(), all.size())) != kOk) return st;
         eval::GridDesc::BlkStages& sd = desc.blk[e];
         sd.seg_begin = t.blk_tabs[e];
         sd.segs = sd.seg_begin + p.seg_begin.size();
-        sd.leaf_lds = sd.segs + p.segs.size();
-        sd.stage_first = sd.leaf_lds + p.leaf_lds.size();
         sd.num_stages = p.num_stages;
     }
     t.host_desc = desc;
@@ -959,7 +990,11 @@ void free_tables(DeviceTables& t) {
     free_dev(t.stage_tabs[1]);
     free_dev(t.blk_tabs[0]);
     free_dev(t.blk_tabs[1]);
-    free_dev(t.walk_lsum);
+    for (auto& pk : t.probe)
+        for (auto& pr : pk) {
+            free_dev(pr.info);
+            free_dev(pr.glob);
+        }
     if (t.desc) (void)hipFree(t.desc);
     t.desc = nullptr;
     for (auto& dm : t.dims) {

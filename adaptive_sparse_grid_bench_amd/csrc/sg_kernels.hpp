@@ -40,12 +40,9 @@ struct GridDesc {
     struct BlkStages {
         const int32_t* seg_begin;
         const int32_t* segs;
-        const int32_t* leaf_lds;      // the staged-pattern probe's tables
-        const int32_t* stage_first;
         int32_t num_stages;
         int32_t pad;
     } blk[2];
-    const uint8_t* leaf_lsum;     // level sum of every leaf in walk order (the probes)
 };
 
 // A run of at most one workgroup's points of one grid.
@@ -69,7 +66,14 @@ struct DeviceTables {
     uint8_t* sub_lsum = nullptr;    // [S] level sum of each subspace (compact order)
     int32_t* dfs_base = nullptr;    // [S]
     uint8_t* dfs_bound = nullptr;   // explicit sets: walk-node bounds
-    uint8_t* walk_lsum = nullptr;   // [S] level sum of each leaf in walk order
+    // tables of the staged-pattern probe per plan ([0] staged, [1] block-staged) and
+    // element size: per stage (staged leaves, cell mask), per gathered leaf (base, level sum)
+    struct Probe {
+        int32_t* info = nullptr;
+        int32_t* glob = nullptr;
+        int num_stages = 0;
+        int num_glob = 0;
+    } probe[2][2];
     int32_t* stage_tabs[2] = {};    // StagePlan arrays per element size, one allocation each
     int32_t* blk_tabs[2] = {};      // BlockStagePlan arrays per element size
     void* desc = nullptr;           // the grid's evaluation descriptor (eval::GridDesc)

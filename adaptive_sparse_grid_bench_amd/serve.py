@@ -23,7 +23,7 @@ import time
 
 import numpy as np
 
-from . import sgrid
+from . import autotune, sgrid
 from . import workloads as W
 
 
@@ -74,6 +74,7 @@ def make_grids(count: int = 64, seed: int = 21, adaptive_every: int = 4,
             g = sgrid.Grid(d, n)
             if g.N > max_points:
                 continue
+            autotune.apply_cached(g)             # the server takes each grid's variant
             kind = "regular"
             centre = None
         x = torch.empty((g.N, g.d), dtype=torch.float64, device="cuda")

@@ -121,40 +121,61 @@ int check_blk_plan(const sg::Grid& g, int e) {
     struct Leaf { int64_t base, bytes; int lds; };
     std::vector<std::vector<Leaf>> stage_leaves;
     int used = 0, buf = -1;
-    int64_t staged = 0, total = 0;
+    int64_t staged = 0, total = 0, sliced = 0;
+    auto place = [&](int64_t bytes) -> int {
+        if (buf < 0 || used + bytes > STB) {
+            buf = buf < 0 ? 0 : buf ^ 1;
+            used = 0;
+            stage_leaves.emplace_back();
+        }
+        const int at = buf * STB + used;
+        used += int(bytes);
+        return at;
+    };
     for (const auto& outer : w.blocks) {
         int H = 0;
         for (int k = J; k < g.d; ++k) H += outer[k];
         const int R = nm1 - H;
         const int64_t C = sg::stage_binom(R + J, J);
         total += C;
-        const int64_t bytes = sg::blk_bytes(J, R, H, elem);
-        if (bytes > STB) continue;
-        if (buf < 0 || used + bytes > STB) {
-            buf = buf < 0 ? 0 : buf ^ 1;
-            used = 0;
-            stage_leaves.emplace_back();
-        }
-        // leaves of the block: inner levels with sum <= R
-        std::vector<int> l(outer);
+        const int how = sg::blk_staging(J, R, H, elem, STB);
+        if (how == 0) continue;
+        // leaves of the block (inner levels with sum <= R) in walk order, dim J-1 slowest
+        std::vector<std::vector<int>> leaves;
         std::vector<int> v(J, 0);
         for (;;) {
             int s = 0;
-            for (int j = 0; j < J; ++j) { l[j] = v[j]; s += v[j]; }
-            if (s <= R) {
-                const int64_t id = sg::subspace_id(g, l.data());
-                if (id < 0) return fail("blk leaf not in the grid", g.d, elem, -1);
-                int inner[sg::kMaxDims] = {};
-                for (int j = 0; j < J; ++j) inner[j] = v[j];
-                const int lds = buf * STB + used + int(sg::blk_leaf_offset(J, inner, H, elem));
-                stage_leaves.back().push_back({g.sub_base[id], (int64_t(1) << (H + s)) * elem, lds});
-                ++staged;
-            }
+            for (int j = 0; j < J; ++j) s += v[j];
+            if (s <= R) leaves.push_back(v);
             int j = 0;
             while (j < J && ++v[j] > R) v[j++] = 0;
             if (j == J) break;
         }
-        used += int(bytes);
+        auto add = [&](const std::vector<int>& inner_v, int Jb, int Hb, int at) -> int {
+            std::vector<int> l(outer);
+            int s = 0, inner[sg::kMaxDims] = {};
+            for (int j = 0; j < J; ++j) l[j] = inner_v[j];
+            for (int j = 0; j < Jb; ++j) { inner[j] = inner_v[j]; s += inner_v[j]; }
+            const int64_t id = sg::subspace_id(g, l.data());
+            if (id < 0) return 1;
+            const int lds = at + int(sg::blk_leaf_offset(Jb, inner, Hb, elem));
+            stage_leaves.back().push_back({g.sub_base[id], (int64_t(1) << (Hb + s)) * elem, lds});
+            ++staged;
+            return 0;
+        };
+        if (how == 1) {
+            const int at = place(sg::blk_bytes(J, R, H, elem));
+            for (const auto& lv : leaves)
+                if (add(lv, J, H, at)) return fail("blk leaf not in the grid", g.d, elem, -1);
+        } else {
+            ++sliced;
+            for (int a = 0; a <= R; ++a) {
+                const int at = place(sg::blk_bytes(J - 1, R - a, H + a, elem));
+                for (const auto& lv : leaves)
+                    if (lv[J - 1] == a && add(lv, J - 1, H + a, at))
+                        return fail("blk leaf not in the grid", g.d, elem, -1);
+            }
+        }
     }
     if (total != g.S) return fail("blk walk does not cover the grid", g.d, elem, -1);
     if (int(stage_leaves.size()) != K) return fail("blk stage count", g.d, elem, -1);
@@ -186,9 +207,9 @@ int check_blk_plan(const sg::Grid& g, int e) {
                     return fail("blk leaf bytes differ from its surpluses", g.d, elem, k);
         }
     }
-    std::printf("d=%d n=%d elem=%d block-staged: %d stages, %lld full + %lld partial segments, %lld of %lld leaves staged: ok\n",
+    std::printf("d=%d n=%d elem=%d block-staged: %d stages, %lld full + %lld partial segments, %lld of %lld leaves staged (%lld blocks by slices): ok\n",
                 g.d, g.n, elem, K, (long long)full, (long long)partial, (long long)staged,
-                (long long)g.S);
+                (long long)g.S, (long long)sliced);
     return 0;
 }
 

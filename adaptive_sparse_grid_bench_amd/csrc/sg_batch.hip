@@ -183,7 +183,9 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
     }
     // descriptors and work lists, grouped by (explicit set, dimensions, table length)
     std::vector<eval::GridDesc> desc(G);
-    std::map<std::tuple<int, int, int, int, int>, std::vector<eval::WorkItem>> groups_h;
+    // key: explicit set, d, table length, FP64 and FP32 variants, and n - 1 where a
+    // variant is the block-staged walk (its kernels are built per n)
+    std::map<std::tuple<int, int, int, int, int, int>, std::vector<eval::WorkItem>> groups_h;
     for (int i = 0; i < G; ++i) {
         const Grid& g = *grids[i].g;
         eval::GridDesc& D = desc[i];
@@ -194,9 +196,10 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
         D.out_off = point_off[i];
         // table length valid for every variant (variant 0 tables the most dimensions)
         const int lt = eval::table_levels(g, 0);
-        auto& items = groups_h[std::make_tuple(g.explicit_set ? 1 : 0, g.d, lt,
-                                               work_list_variant(g, grids[i].variant_f64, 8),
-                                               work_list_variant(g, grids[i].variant_f32, 4))];
+        const int v64 = work_list_variant(g, grids[i].variant_f64, 8);
+        const int v32 = work_list_variant(g, grids[i].variant_f32, 4);
+        const int nm1 = (v64 == kBlkVariant || v32 == kBlkVariant) ? g.n - 1 : -1;
+        auto& items = groups_h[std::make_tuple(g.explicit_set ? 1 : 0, g.d, lt, v64, v32, nm1)];
         for (int64_t f = 0; f < counts[i]; f += eval::kThreads)
             items.push_back({i, int32_t(f), int32_t(std::min<int64_t>(eval::kThreads, counts[i] - f)), 0});
     }
@@ -209,6 +212,7 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
         gr.lt = std::get<2>(kv.first);
         gr.variant_f64 = std::get<3>(kv.first);
         gr.variant_f32 = std::get<4>(kv.first);
+        gr.nm1 = std::get<5>(kv.first);
         gr.first = int64_t(all.size());
         gr.count = int64_t(kv.second.size());
         all.insert(all.end(), kv.second.begin(), kv.second.end());
@@ -248,7 +252,9 @@ int Batch::evaluate_group(const Group& gr, const T* alpha, const T* coords, T* o
     prm.perm = nullptr;
     prm.P = 0;
     const int variant = sizeof(T) == 4 ? gr.variant_f32 : gr.variant_f64;
-    hipError_t e = gr.explicit_set
+    hipError_t e = variant == kBlkVariant
+        ? eval::launch_d_blk<T, Acc>(gr.d, gr.nm1, prm, gr.count, s)
+        : gr.explicit_set
         ? eval::launch_d<true, T, Acc>(gr.d, variant == kStagedVariant ? variant : 0, gr.lt,
                                        prm, gr.count, eval::kThreads, s)
         : eval::launch_d<false, T, Acc>(gr.d, variant, gr.lt, prm, gr.count, eval::kThreads, s);

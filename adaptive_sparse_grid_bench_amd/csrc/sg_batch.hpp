@@ -39,6 +39,7 @@ struct Batch {
         int lt;
         int variant_f64;
         int variant_f32;
+        int nm1;                 // n - 1 of the grids (block-staged groups), else -1
         int64_t first;           // first work item
         int64_t count;           // work items (= workgroups)
     };

@@ -152,11 +152,14 @@ inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
     if (variant == kStagedVariant && !g.stage[e].ok) return 0;
     return variant;
 }
-// the variant of a work-list launch (batches, the server): the block-staged walk runs
-// one grid per launch, so work lists take the staged walk in its place
+// the variant of a work-list launch (batches, the server): work items hold up to
+// eval::kThreads points, the workgroup size of every walk there; a block-staged grid
+// whose workgroups differ in size runs the staged walk instead
 inline int work_list_variant(const Grid& g, int variant, int elem_bytes) {
     const int v = effective_variant(g, variant, elem_bytes);
-    return v == kBlkVariant ? effective_variant(g, kStagedVariant, elem_bytes) : v;
+    if (v == kBlkVariant && blk_threads(elem_bytes) != 256)
+        return effective_variant(g, kStagedVariant, elem_bytes);
+    return v;
 }
 
 // perm: lane order of the points (from sort_points) or null for the given order

@@ -208,8 +208,8 @@ struct Server::Impl {
         }
         lk.unlock();
         // a grid with a pending injected failure gets a launch group of its own
-        std::map<std::tuple<int, int, int, int, int>, std::vector<eval::WorkItem>> groups;
-        std::map<std::tuple<int, int, int, int, int>, std::vector<size_t>> group_reqs;
+        std::map<std::tuple<int, int, int, int, int, int>, std::vector<eval::WorkItem>> groups;
+        std::map<std::tuple<int, int, int, int, int, int>, std::vector<size_t>> group_reqs;
         int64_t coff = 0, ooff = 0;
         std::vector<int64_t> out_off(rs.size());
         for (size_t i = 0; i < rs.size(); ++i) {
@@ -225,7 +225,8 @@ struct Server::Impl {
             D.coord_off = coff;
             D.out_off = ooff;
             const auto key = std::make_tuple(g.explicit_set ? 1 : 0, g.d, eval::table_levels(g, 0),
-                                             ge.variant, inject_of[i] ? r.grid : -1);
+                                             ge.variant, inject_of[i] ? r.grid : -1,
+                                             ge.variant == kBlkVariant ? g.n - 1 : -1);
             auto& items = groups[key];
             group_reqs[key].push_back(i);
             for (int64_t f = 0; f < r.P; f += eval::kThreads)
@@ -237,7 +238,7 @@ struct Server::Impl {
         }
         int64_t nitems = 0;
         struct LaunchGroup {
-            int expl, d, lt, variant;
+            int expl, d, lt, variant, nm1;
             bool injected;
             int64_t first, count;
             const std::vector<size_t>* reqs;
@@ -247,8 +248,8 @@ struct Server::Impl {
             std::memcpy(s.h_work + nitems, kv.second.data(), kv.second.size() * sizeof(eval::WorkItem));
             launches_v.push_back({std::get<0>(kv.first), std::get<1>(kv.first),
                                   std::get<2>(kv.first), std::get<3>(kv.first),
-                                  std::get<4>(kv.first) >= 0, nitems, int64_t(kv.second.size()),
-                                  &group_reqs[kv.first]});
+                                  std::get<5>(kv.first), std::get<4>(kv.first) >= 0, nitems,
+                                  int64_t(kv.second.size()), &group_reqs[kv.first]});
             nitems += int64_t(kv.second.size());
         }
         hipError_t e = hipMemcpyAsync(s.d_coords, s.h_coords, size_t(coff) * elem,
@@ -267,7 +268,7 @@ struct Server::Impl {
             if (e != hipSuccess) break;
             const hipError_t le = L.injected
                 ? hipErrorLaunchFailure
-                : launch_group(L.expl != 0, L.d, L.lt, L.variant, L.first, L.count, s);
+                : launch_group(L.expl != 0, L.d, L.lt, L.variant, L.nm1, L.first, L.count, s);
             if (le != hipSuccess) {
                 (void)hipGetLastError();               // clear the launch error
                 launch_error = std::string("server launch") + (L.injected ? " (injected)" : "") +
@@ -323,8 +324,8 @@ struct Server::Impl {
         return true;
     }
 
-    hipError_t launch_group(bool expl, int d, int lt, int variant, int64_t first, int64_t count,
-                            Slot& s) {
+    hipError_t launch_group(bool expl, int d, int lt, int variant, int nm1, int64_t first,
+                            int64_t count, Slot& s) {
         if (elem == 8) {
             eval::Params<double> prm{};
             prm.alpha = nullptr;                       // descriptors carry the addresses
@@ -332,6 +333,8 @@ struct Server::Impl {
             prm.out = static_cast<double*>(s.d_out);
             prm.grids = s.d_desc;
             prm.work = s.d_work + first;
+            if (variant == kBlkVariant)
+                return eval::launch_d_blk<double, eval::AccF64>(d, nm1, prm, count, s.stream);
             return expl ? eval::launch_d<true, double, eval::AccF64>(d, variant, lt, prm, count,
                                                                      eval::kThreads, s.stream)
                         : eval::launch_d<false, double, eval::AccF64>(
@@ -343,6 +346,8 @@ struct Server::Impl {
         prm.out = static_cast<float*>(s.d_out);
         prm.grids = s.d_desc;
         prm.work = s.d_work + first;
+        if (variant == kBlkVariant)
+            return eval::launch_d_blk<float, eval::AccKahan>(d, nm1, prm, count, s.stream);
         return expl ? eval::launch_d<true, float, eval::AccKahan>(d, variant, lt, prm, count,
                                                                   eval::kThreads, s.stream)
                     : eval::launch_d<false, float, eval::AccKahan>(

@@ -150,3 +150,55 @@ def test_stage_probe_reads_once_per_subspace(sg, torch_gpu, d, n, prec, variant)
         assert np.allclose(got, want, rtol=g.S * 2.0 ** -24, atol=0)
     else:
         assert np.array_equal(got, want), float(np.max(np.abs(got - want)))
+
+
+def test_blk_in_batches_and_the_server(sg, torch_gpu):
+    """Work-list launches (batches, their graph, the server) run the block-staged walk
+    for the grids tuned to it, grouped by n; outputs equal the per-grid calls."""
+    torch = torch_gpu
+    shapes = [(12, 5), (11, 4), (9, 6), (12, 6), (8, 5), (3, 4)]
+    grids, vals = [], []
+    for d, n in shapes:
+        sh = O.tables(d, n)
+        g = sg.Grid(d, n)
+        g.set_tuning(eval_variant_f64=BLK, eval_variant_f32=BLK)
+        grids.append(g)
+        vals.append(W.f_gauss(O.grid_coords(sh)))
+    counts = [300, 1000, 257, 64, 700, 5]
+    b = sg.Batch(grids)
+    b.set_points(counts)
+    nodal = torch.from_numpy(np.concatenate(vals)).cuda()
+    coords = torch.empty(int(b.coord_offsets[-1]), dtype=torch.float64, device="cuda")
+    sg.uniform_points(coords, seed=21)
+    for dt in (torch.float64, torch.float32):
+        a = nodal.to(dt).clone()
+        b.hierarchize(a)
+        out = torch.full((int(b.point_offsets[-1]),), float("nan"), dtype=dt, device="cuda")
+        b.run_graph(a.clone(), coords.to(dt), out, hierarchize=False)
+        torch.cuda.synchronize()
+        for i, g in enumerate(grids):
+            lo, hi = int(b.value_offsets[i]), int(b.value_offsets[i + 1])
+            c0, c1 = int(b.coord_offsets[i]), int(b.coord_offsets[i + 1])
+            p0, p1 = int(b.point_offsets[i]), int(b.point_offsets[i + 1])
+            oi = torch.empty(p1 - p0, dtype=dt, device="cuda")
+            g.evaluate(a[lo:hi], coords.to(dt)[c0:c1].view(p1 - p0, g.d), oi)
+            assert torch.equal(oi, out[p0:p1]), (i, dt)
+        prec = "f64" if dt == torch.float64 else "f32c"
+        srv = sg.Server(prec, slots=2, max_batch_points=1 << 13, max_requests=32)
+        ids = [srv.add_grid(g, a[int(b.value_offsets[i]):int(b.value_offsets[i + 1])])
+               for i, g in enumerate(grids)]
+        npdt = np.float64 if dt == torch.float64 else np.float32
+        reqs = []
+        for k in range(18):
+            i = k % len(grids)
+            pts = W.uniform_points(50 + 37 * k, grids[i].d, seed=300 + k).astype(npdt)
+            o = np.full(len(pts), np.nan, npdt)
+            reqs.append((i, pts, o, srv.submit(ids[i], pts, o)))
+        for i, pts, o, rid in reqs:
+            srv.wait(rid)
+            g = grids[i]
+            ref = torch.empty(len(pts), dtype=dt, device="cuda")
+            g.evaluate(a[int(b.value_offsets[i]):int(b.value_offsets[i + 1])],
+                       torch.from_numpy(pts).cuda(), ref)
+            assert np.array_equal(o, ref.cpu().numpy()), (i, prec)
+        srv.close()

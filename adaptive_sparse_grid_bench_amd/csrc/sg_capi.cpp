@@ -86,8 +86,11 @@ int sort_one(const float* x, int64_t P, int d, int kb, void* ws, uint32_t* perm,
     return sg::launch_sort_points_f32(x, P, d, kb, ws, perm, s);
 }
 
+// ws / ws_bytes: a fixed workspace for the binning (the allocation-free entry point), or
+// null for stream-ordered memory allocated and freed per call.
 template <class T>
-int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, void* stream) {
+int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, void* stream,
+                    void* ws = nullptr, size_t ws_cap = 0) {
     hipStream_t s = as_stream(stream);
     // binning pays when the walk per point is long: skip it for small grids, and for the
     // staged walk, where it only removes LDS bank conflicts: at d=10, n=8 and 2^24 points
@@ -99,6 +102,13 @@ int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, v
         return eval_one(h, a, x, P, o, nullptr, s);
     const int kb = sg::key_bits_for(P, h->g.d);
     const size_t ws_bytes = sg::sort_workspace_bytes(P, kb);
+    if (ws) {
+        if (ws_bytes + size_t(P) * 4 > ws_cap)
+            return fail(sg::kRange, "evaluate: binning workspace exceeds the fixed scratch");
+        uint32_t* perm = reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + ws_bytes);
+        int st = sort_one(x, P, h->g.d, kb, ws, perm, s);
+        return st == sg::kOk ? eval_one(h, a, x, P, o, perm, s) : st;
+    }
     void* mem = nullptr;
     hipError_t e = hipMallocAsync(&mem, ws_bytes + size_t(P) * 4, s);
     if (e != hipSuccess)
@@ -320,6 +330,35 @@ int sg_evaluate_f64(const sg_grid* grid, const double* d_alpha, const double* d_
                     long long num_evals, double* d_out, void* stream) {
     SG_EVAL_CHECK("sg_evaluate_f64")
     return evaluate_binned<double>(grid, d_alpha, d_coords, num_evals, d_out, stream);
+}
+
+int sg_evaluate_f64_noalloc(const sg_grid* grid, const double* d_alpha, const double* d_coords,
+                            long long num_evals, double* d_out, void* stream) {
+    if (!grid || num_evals < 0 || (num_evals > 0 && (!d_alpha || !d_coords || !d_out)))
+        return fail(sg::kInvalid, "sg_evaluate_f64_noalloc: bad argument");
+    if (num_evals > sg::kNoAllocMaxPoints)
+        return fail(sg::kRange, "sg_evaluate_f64_noalloc: " + std::to_string(num_evals) +
+                                    " points exceed SG_NOALLOC_MAX_POINTS (" +
+                                    std::to_string(sg::kNoAllocMaxPoints) + ")");
+    sg_grid* h = const_cast<sg_grid*>(grid);
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return fail(sg::kHip, "hipGetDevice failed");
+    {
+        // uploading the grid's tables allocates: here they must already be on the device
+        std::lock_guard<std::mutex> lock(h->mu);
+        if (!h->uploaded)
+            return fail(sg::kInvalid,
+                        "sg_evaluate_f64_noalloc: the grid's device tables are not uploaded; "
+                        "make one other device call on the grid first (e.g. "
+                        "sg_hierarchize_f64 or sg_evaluate_f64)");
+        if (cur != h->t.device)
+            return fail(sg::kInvalid, "current device differs from the grid's device");
+    }
+    void* ws = nullptr;
+    size_t ws_cap = 0;
+    int st = sg::noalloc_scratch(&ws, &ws_cap);
+    if (st != sg::kOk) return st;
+    return evaluate_binned<double>(grid, d_alpha, d_coords, num_evals, d_out, stream, ws, ws_cap);
 }
 
 int sg_evaluate_f32c(const sg_grid* grid, const float* d_alpha, const float* d_coords,

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "sg_kernels.hpp"
 
@@ -1062,6 +1063,41 @@ int key_bits_for(int64_t P, int d) {
 
 size_t sort_workspace_bytes(int64_t P, int key_bits) {
     return size_t(P) * 4 + ((size_t(1) << key_bits) + 1) * 4;
+}
+
+// Scratch of the allocation-free evaluate entry point: the binning workspace (keys and
+// key histogram) followed by the point permutation, for up to kNoAllocMaxPoints points.
+// Every device holds its own copy of the array for the life of the process.  Host code
+// uses the address hipGetSymbolAddress returns on that device, never the array's name
+// (which is a host-side stand-in).
+constexpr size_t kNoAllocScratchBytes =
+    (size_t(kNoAllocMaxPoints) * 4 + ((size_t(1) << 16) + 1) * 4 + size_t(kNoAllocMaxPoints) * 4 +
+     255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+int noalloc_scratch(void** ptr, size_t* bytes) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = -1;
+    SG_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("noalloc scratch: device ordinal out of range");
+        return kInvalid;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        SG_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("noalloc scratch: hipGetSymbolAddress returned no address");
+            return kHip;
+        }
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    *bytes = kNoAllocScratchBytes;
+    return kOk;
 }
 
 int launch_sort_points_f64(const double* coords, int64_t P, int d, int key_bits, void* ws,

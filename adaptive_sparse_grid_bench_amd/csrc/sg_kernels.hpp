@@ -180,6 +180,13 @@ int launch_sort_points_f64(const double* coords, int64_t P, int d, int key_bits,
                            uint32_t* perm, hipStream_t s);
 int launch_sort_points_f32(const float* coords, int64_t P, int d, int key_bits, void* ws,
                            uint32_t* perm, hipStream_t s);
+
+// Fixed device scratch of the allocation-free evaluate (sg_evaluate_f64_noalloc): the
+// current device's address of a file-scope device array holding the binning workspace
+// and permutation of up to kNoAllocMaxPoints points (looked up once per device).
+constexpr int64_t kNoAllocMaxPoints = int64_t(1) << 21;
+int noalloc_scratch(void** ptr, size_t* bytes);
+
 int launch_evaluate_ref_f64(const Grid& g, const DeviceTables& t, const double* alpha,
                             const double* coords, int64_t P, double* out, hipStream_t s);
 int launch_evaluate_ref_f32(const Grid& g, const DeviceTables& t, const float* alpha,

@@ -39,7 +39,7 @@ EXPORTS = [
     "sg_idx2gp_dev", "sg_gp2idx_dev", "sg_grid_coords_f64",
     "sg_hierarchize_f64", "sg_hierarchize_f32",
     "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
-    "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
+    "sg_evaluate_f64_noalloc", "sg_uniform_points_f64", "sg_uniform_points_f32", "sg_convert_f64_to_f32",
     "sg_gather_probe_f64", "sg_gather_probe_f32", "sg_stage_probe_f64", "sg_stage_probe_f32",
     "sg_sample_f64", "sg_sample_f32",
     "sg_file_write", "sg_file_info", "sg_file_read",
@@ -119,6 +119,7 @@ def _load() -> C.CDLL:
         "sg_evaluate_f32c": (i, [vp, vp, vp, ll, vp, vp]),
         "sg_evaluate_ref_f64": (i, [vp, vp, vp, ll, vp, vp]),
         "sg_evaluate_ref_f32": (i, [vp, vp, vp, ll, vp, vp]),
+        "sg_evaluate_f64_noalloc": (i, [vp, vp, vp, ll, vp, vp]),
         "sg_uniform_points_f64": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_uniform_points_f32": (i, [ll, C.c_ulonglong, vp, vp]),
         "sg_convert_f64_to_f32": (i, [vp, vp, ll, vp]),

@@ -158,6 +158,23 @@ int sg_evaluate_ref_f64(const sg_grid *grid, const double *d_alpha, const double
 int sg_evaluate_ref_f32(const sg_grid *grid, const float *d_alpha, const float *d_coords,
                         long long num_evals, float *d_out, void *stream);
 
+/* sg_evaluate_f64 without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The point-binning workspace that
+ * sg_evaluate_f64 takes from stream-ordered allocations comes from a fixed device array
+ * of the library instead (about 16.3 MiB, held by each device the library runs on until
+ * the process ends), so:
+ *   - num_evals is at most SG_NOALLOC_MAX_POINTS, else SG_E_RANGE;
+ *   - the grid's device tables must already be uploaded by an earlier device call on
+ *     the grid (any of sg_hierarchize_*, sg_evaluate_*, ...), else SG_E_INVALID;
+ *   - every call on one device shares that device's array: calls that overlap on one
+ *     device (different streams, or host threads) race and give wrong results.
+ * Results are bit-identical to sg_evaluate_f64's. */
+#define SG_NOALLOC_MAX_POINTS 2097152LL
+int sg_evaluate_f64_noalloc(const sg_grid *grid, const double *d_alpha,
+                            const double *d_coords, long long num_evals, double *d_out,
+                            void *stream);
+
 /* ---------------------------------------------------------- batches --------- */
 /* A batch of independent grids resident together (the batched form).  Surpluses of
  * grid g live at value_offsets[g] of one buffer; after sg_batch_set_points(counts),

@@ -73,6 +73,19 @@ def test_host_errors():
     assert g.get_tuning()["eval_block"] == 128 and g.get_tuning()["bin_min_points"] == -1
 
 
+def test_noalloc_evaluate_argument_checks():
+    """sg_evaluate_f64_noalloc refuses bad arguments and point counts beyond its fixed
+    scratch before it touches a device."""
+    from adaptive_sparse_grid_bench_amd import sgrid
+    g = sgrid.Grid(3, 4)
+    fn = sgrid.lib.sg_evaluate_f64_noalloc
+    assert fn(None, 8, 8, 4, 8, None) == sgrid.SG_E_INVALID
+    assert fn(g._h, None, 8, 4, 8, None) == sgrid.SG_E_INVALID
+    assert fn(g._h, 8, 8, -1, 8, None) == sgrid.SG_E_INVALID
+    assert fn(g._h, 8, 8, 2097152 + 1, 8, None) == sgrid.SG_E_RANGE
+    assert "SG_NOALLOC_MAX_POINTS" in sgrid.lib.sg_last_error().decode()
+
+
 def test_explicit_set_matches_restricted_regular():
     """A restricted-regular level set given explicitly gets the identical compact order."""
     from adaptive_sparse_grid_bench_amd import sgrid

@@ -757,6 +757,89 @@ struct WalkerBlk {
     }
 };
 
+// Lane state of the sparse block-staged walk: the J tabled dimensions' hat / offset
+// tables.  The coordinates of the outer dimensions J..D-1, which the walk indexes with a
+// wave-uniform dimension known only at run time, live in separate kernel-local arrays of
+// 32-bit words (FP64: low and high halves): those the compiler indexes in registers
+// (s_set_gpr_idx), where an array of doubles, or a struct member, indexed at run time
+// would keep its whole object in scratch memory.
+template <int J, class T>
+struct LaneSp {
+    T ph[J][8];
+    uint32_t ub[J][8];
+};
+template <class T>
+__device__ __forceinline__ T coord_at(const uint32_t* lo, const uint32_t* hi, int i) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, (uint64_t(hi[i]) << 32) | lo[i]);
+    else
+        return __builtin_bit_cast(float, lo[i]);
+}
+
+// The sparse order of the block-staged walk (blk_sparse; the host's sparse_walk lists the
+// same blocks in the same order).  A node at nesting depth DEPTH has its dimensions >= c
+// fixed, with P the product of their hats (DEPTH of them nonzero, so its budget rem is
+// at most NM1 - DEPTH).  For k = c-1 down to J it visits the children with l_k = 1..rem,
+// each with weight W * hat_k(l_k), where W = P * hat_{c-1}(0) * ... * hat_{k+1}(0) grows by
+// one level-0 hat per k (also when rem = 0: the node's block needs all of them); then its
+// own static block (dimensions J..c-1 at level 0) with the full product W.  Each weight is the product of the same hats, in the same
+// order (outer dimensions descending), as in the depth-first walk.  Level 0 adds nothing
+// to the in-block offset or the level sum, so dimension k's index bits shift by hsum, the
+// sum of the nonzero levels above it.
+// A node with no budget left (most nodes at high d: 20,349 of 26,334 at d = 20, n = 6)
+// has a one-leaf block with every dimension 0..c-1 at level 0; its weight takes the
+// per-point product Z_c = hat_0(0) * ... * hat_{c-1}(0) from a table instead of c - J
+// steps of the loop and J multiplies in the block.  That product's rounding differs from
+// the loop's (so from the depth-first walk's), well inside the FP64 bound.
+template <int J, int NM1, int B, class Ln, class T, class St>
+__device__ __forceinline__ void single_leaf(Ln& ln, T w, uint32_t off, int hsum, St& st) {
+    constexpr int E = int(sizeof(T));
+    if constexpr (blk_staging(J, 0, NM1, E, St::kStageBytes) == 1) {
+        // the leaf (0, ..., 0) sits at the start of its block's LDS image
+        st.template block_begin<uint32_t(blk_bytes(J, 0, NM1, E))>();
+        st.acc.add(w, st.fetch_lds_at(off + st.base));
+        st.q += 1;
+    } else {
+        st.acc.add(w, st.fetch_global(st.q, off));
+        st.q += 1;
+    }
+}
+
+template <int DEPTH, int NM1, int J, int B, class T, class St, class Ln>
+struct WalkerSp {
+    // xlo / xhi: coordinates of dimensions J.., zlo / zhi: Z_J..Z_D (coord_at); rem >= 1
+    static __device__ __forceinline__ void run(Ln& ln, const uint32_t* xlo, const uint32_t* xhi,
+                                               const uint32_t* zlo, const uint32_t* zhi, int c,
+                                               T P, uint32_t off, int hsum, int rem, St& st) {
+        T W = P;
+        for (int k = c - 1; k >= J; --k) {
+            const T xk = coord_at<T>(xlo, xhi, k - J);
+            for (int l = 1; l <= rem; ++l) {
+                T phi;
+                uint32_t t;
+                hat_at(xk, l, phi, t);
+                const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
+                if (l == rem) {
+                    // the child has no budget left: its one-leaf block, every tabled
+                    // dimension at level 0 (in-block offset 0, hats folded into Z)
+                    single_leaf<J, NM1, B>(ln, (W * phi) * coord_at<T>(zlo, zhi, k - J), o,
+                                           hsum + l, st);
+                } else {
+                    if constexpr (DEPTH + 1 < NM1)
+                        WalkerSp<DEPTH + 1, NM1, J, B, T, St, Ln>::run(ln, xlo, xhi, zlo, zhi, k,
+                                                                       W * phi, o, hsum + l,
+                                                                       rem - l, st);
+                }
+            }
+            T phi0;
+            uint32_t t0;
+            hat_at(xk, 0, phi0, t0);
+            W = W * phi0;
+        }
+        static_block_dispatch<J, NM1 - DEPTH + 1, B>(rem, ln, W, off, hsum, st);
+    }
+};
+
 // Depth-first walk.  Dimension K's level runs 0..bound, where bound = min(rem,
 // limit_K) for a restricted-regular set and comes from the node-code stream for an
 // explicit set.  Dimension d-1 is outermost, so for a fixed level sum the leaves come
@@ -856,9 +939,16 @@ void k_eval(Params<T> prm) {
     // permutation, neighbouring lanes hold spatially close points, so a wave's gathers
     // share cache lines
     const int64_t pc = live ? (prm.perm ? int64_t(prm.perm[p]) : p) : first;
-    constexpr bool kLean = NM1 >= 0 && blk_lean_walk<T>();
-    using Ln = std::conditional_t<kLean, LaneBlk<D, J, T>, Lane<D, LT, J, T>>;
+    constexpr bool kSparse = NM1 >= 0 && J > 0 && D > J && blk_sparse(D, int(sizeof(T)));
+    constexpr bool kLean = NM1 >= 0 && !kSparse && blk_lean_walk<T>();
+    using Ln = std::conditional_t<kSparse, LaneSp<(J > 0 ? J : 1), T>,
+                                  std::conditional_t<kLean, LaneBlk<D, J, T>, Lane<D, LT, J, T>>>;
     Ln ln;
+    // sparse walk: coordinates of dimensions J.. and the level-0 hat products Z_c, c =
+    // J..D (single_leaf), as 32-bit words (coord_at)
+    constexpr int kXs = kSparse ? D - J : 1;
+    uint32_t xlo[kXs], xhi[sizeof(T) == 8 ? kXs : 1];
+    uint32_t zlo[kXs + 1], zhi[sizeof(T) == 8 ? kXs + 1 : 1];
     const T* x = prm.coords + G->coord_off + pc * D;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -866,11 +956,45 @@ void k_eval(Params<T> prm) {
         if constexpr (J > 0) {
             if (k < J) level_table<8, T>(xk, ln.ph[k < J ? k : 0], ln.ub[k < J ? k : 0]);
         }
-        if constexpr (kLean) ln.x[k] = xk;
-        else ln.m0[k] = T(2) * xk - T(1);
+        if constexpr (kSparse) {
+            if (k >= J) {
+                if constexpr (sizeof(T) == 8) {
+                    const uint64_t b = __builtin_bit_cast(uint64_t, xk);
+                    xlo[k >= J ? k - J : 0] = uint32_t(b);
+                    xhi[k >= J ? k - J : 0] = uint32_t(b >> 32);
+                } else {
+                    xlo[k >= J ? k - J : 0] = __builtin_bit_cast(uint32_t, xk);
+                }
+            }
+        } else if constexpr (kLean) {
+            ln.x[k] = xk;
+        } else {
+            ln.m0[k] = T(2) * xk - T(1);
+        }
+    }
+    if constexpr (kSparse) {
+        T z = T(1);
+#pragma unroll
+        for (int j = 0; j < J; ++j) z = z * ln.ph[j][0];
+#pragma unroll
+        for (int i = 0; i <= kXs; ++i) {
+            if constexpr (sizeof(T) == 8) {
+                const uint64_t b = __builtin_bit_cast(uint64_t, z);
+                zlo[i] = uint32_t(b);
+                zhi[i] = uint32_t(b >> 32);
+            } else {
+                zlo[i] = __builtin_bit_cast(uint32_t, z);
+            }
+            if (i < kXs) {
+                T phi0;
+                uint32_t t0;
+                hat_at(x[J + (i < kXs ? i : 0)], 0, phi0, t0);
+                z = z * phi0;
+            }
+        }
     }
     int lim[D];
-    if constexpr (!kLean) {
+    if constexpr (!kLean && !kSparse) {
 #pragma unroll
         for (int k = 0; k < D; ++k) lim[k] = G->limits[k];
     }
@@ -891,6 +1015,7 @@ void k_eval(Params<T> prm) {
         st.lds = (lds_byte*)blk_buf;
         st.seg_begin = as_const(tb->seg_begin);
         st.segs = as_const(tb->segs);
+        st.leaf_base = as_const(tb->leaf_base);
         st.num_stages = tb->num_stages;
         st.stage = 0;
         st.base = 0;
@@ -919,7 +1044,10 @@ void k_eval(Params<T> prm) {
             if (st.num_stages > 1) st.issue(1);
         }
     }
-    if constexpr (kLean)
+    if constexpr (kSparse)
+        WalkerSp<0, (NM1 > 0 ? NM1 : 0), J, B, T, St, Ln>::run(ln, xlo, xhi, zlo, zhi, D, T(1),
+                                                               0u, 0, NM1, st);
+    else if constexpr (kLean)
         WalkerBlk<D - 1, D, J, B, T, St>::run(ln, T(1), 0u, 0, NM1, st);
     else
         Walker<D - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);

@@ -2,6 +2,7 @@
 #include "sg_grid.hpp"
 
 #include <array>
+#include <functional>
 
 #include <algorithm>
 #include <cstring>
@@ -503,12 +504,49 @@ void build_stage_plan(const Grid& g, int J, const std::vector<std::array<int32_t
     p.ok = true;
 }
 
+// Static blocks and leaf bases of a regular grid in the sparse order (blk_sparse): the
+// host form of the kernel's sparse walk.  blocks: first leaf, budget R, outer level sum.
+bool sparse_walk(const Grid& g, int J, std::vector<std::array<int32_t, 3>>& blocks,
+                 std::vector<int32_t>& bases) {
+    blocks.clear();
+    bases.clear();
+    int lv[kMaxDims] = {};
+    bool ok = true;
+    const int nm1 = g.n - 1;
+    std::function<void(int, int)> leaves = [&](int k, int R) {     // a static block's leaves
+        for (int l = 0; l <= R; ++l) {
+            lv[k] = l;
+            if (k == 0) {
+                const int64_t id = subspace_id(g, lv);
+                if (id < 0) ok = false;
+                else bases.push_back(g.sub_base[size_t(id)]);
+            } else {
+                leaves(k - 1, R - l);
+            }
+        }
+        lv[k] = 0;
+    };
+    std::function<void(int, int)> node = [&](int c, int rem) {
+        if (rem > 0)
+            for (int k = c - 1; k >= J; --k)
+                for (int l = 1; l <= rem; ++l) {
+                    lv[k] = l;
+                    node(k, rem - l);
+                    lv[k] = 0;
+                }
+        blocks.push_back({int32_t(bases.size()), rem, nm1 - rem});
+        if (J > 0) leaves(J - 1, rem);
+    };
+    node(g.d, nm1);
+    return ok && int64_t(bases.size()) == g.S;
+}
+
 // The block-staged walk's copy plan (see BlockStagePlan): blocks in walk order, each
 // laid out by blk_leaf_offset; a stage takes the next staged blocks while they fit.  The
 // kernel applies the same greedy rule with the same compile-time block sizes, so the plan
 // needs no per-block table: only the copy segments of each stage.  Every leaf's LDS
 // offset is checked here against its HBM layout; a mismatch leaves the plan unusable.
-void build_blk_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 3>>& blocks,
+void build_blk_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 3>>& walk_blocks,
                     int elem, int stage_bytes, BlockStagePlan& p) {
     p = BlockStagePlan();
     if (g.explicit_set || g.N * elem > int64_t(0x7fffffff)) return;
@@ -516,6 +554,14 @@ void build_blk_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 
     p.leaf_lds.assign(size_t(g.S), -1);
     for (int j = 0; j < g.d; ++j)
         if (g.limits[j] != nm1) return;                 // the walk assumes a regular grid
+    std::vector<std::array<int32_t, 3>> sparse_blocks;
+    if (blk_sparse(g.d, elem)) {
+        if (!sparse_walk(g, J, sparse_blocks, p.leaf_base)) return;
+    } else {
+        p.leaf_base = g.dfs_base;
+    }
+    const std::vector<std::array<int32_t, 3>>& blocks =
+        blk_sparse(g.d, elem) ? sparse_blocks : walk_blocks;
     std::vector<int32_t> full, part;
     int used = 0, buf = -1;
     auto close_stage = [&]() {
@@ -550,7 +596,7 @@ void build_blk_plan(const Grid& g, int J, const std::vector<std::array<int32_t, 
         std::vector<std::vector<std::pair<int32_t, int64_t>>> runs(size_t(nm1) + 1);
         int lv[kMaxDims] = {};
         for (int c = c0; c < c1; ++c) {
-            const int32_t base = g.dfs_base[size_t(q0 + c)];
+            const int32_t base = p.leaf_base[size_t(q0 + c)];
             const int64_t id = subspace_of(base);
             if (id < 0) return false;
             int s = 0;

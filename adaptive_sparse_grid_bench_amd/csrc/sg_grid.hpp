@@ -103,12 +103,16 @@ SG_HOST_DEVICE constexpr int blk_stage_bytes(int elem) {
 }
 // the block-staged kernels are built for n - 1 in kBlkMinNM1..kBlkMaxNM1 (the tabled
 // dimensions hold levels 0..7) and for the dimensions where they beat the staged walk
-// (2^22-2^24 points, n = 5..8: FP64 from d = 12 on, up to 1.5x at d = 16..20; FP32 for
-// d = 8..16, up to 12 % at d = 10..12)
+// (2^22-2^24 points, n = 5..8: FP64 from d = 12 on, up to 1.5x at d = 16..20 in the
+// depth-first order, 2.5x in the sparse one; FP32 from d = 8 on, up to 12 % at d = 10..12
+// and 13-22 % at d = 17..20 in the depth-first order)
 constexpr int kBlkMinNM1 = 2;
 constexpr int kBlkMaxNM1 = 7;
+#ifndef SG_BLK32_DMAX
+#define SG_BLK32_DMAX 20
+#endif
 SG_HOST_DEVICE constexpr bool blk_built(int d, int elem) {
-    return elem == 8 ? (d >= 9 && d <= 20) : (d >= 7 && d <= 16);
+    return elem == 8 ? (d >= 9 && d <= 20) : (d >= 7 && d <= SG_BLK32_DMAX);
 }
 
 // bytes of run s (level sum s inside the block) of a static block over J tabled
@@ -156,12 +160,40 @@ SG_HOST_DEVICE constexpr int blk_staging(int J, int R, int H, int elem, int stag
     return 2;
 }
 
+// The block-staged walk visits the static blocks in one of two orders.
+//   depth first (the other walks' order): dimension d-1 outermost, levels ascending;
+//   sparse: from a node whose dimensions >= c are fixed, for k = c-1 down to J and
+//     l = 1..rem the child with l_k = l (dimensions k+1..c-1 at level 0), then the node's
+//     own static block (dimensions J..c-1 all at level 0).  The walk nests only as deep
+//     as the number of nonzero outer levels (at most n - 1) instead of d - J loops, and
+//     visits each outer level vector once; at high d with small n that removes most of
+//     the depth-first walk's loop iterations and the per-level state it spills.
+// Both orders give every leaf the same weight, computed in the same order; only the
+// order in which blocks are summed differs.  blk_sparse names the dimensions that take
+// the sparse order, per element size (host plan and kernel agree on it).
+// Measured (2^22 points, ms, depth-first -> sparse): FP64 d = 12, n = 7: 50.0 -> 43.4;
+// d = 14, n = 6: 39.3 -> 30.9; d = 16, n = 5: 23.3 -> 14.7; d = 20, n = 6: 307.9 -> 185.8;
+// d = 10, n = 8: 35.8 -> 38.7.  FP32 (Kahan) d = 12: 29.3 -> 35.4; d = 14: 25.8 -> 26.3;
+// d = 16: 17.5 -> 12.6; d = 20: 225.4 -> 143.9.
+#ifndef SG_BLK_SPARSE64_DMIN
+#define SG_BLK_SPARSE64_DMIN 12
+#endif
+#ifndef SG_BLK_SPARSE32_DMIN
+#define SG_BLK_SPARSE32_DMIN 15
+#endif
+SG_HOST_DEVICE constexpr bool blk_sparse(int d, int elem) {
+    return d >= (elem == 8 ? SG_BLK_SPARSE64_DMIN : SG_BLK_SPARSE32_DMIN);
+}
+
 // How the block-staged walk copies one grid's blocks into LDS, for one element size.
 struct BlockStagePlan {
     bool ok = false;
     int num_stages = 0;
     std::vector<int32_t> seg_begin;   // [2K+1] as StagePlan::seg_begin
     std::vector<int32_t> segs;        // [nseg + 4][2] as StagePlan::segs
+    // subspace base offset of every leaf in this walk's order (the order blk_sparse
+    // picks); the kernel gathers the leaves of unstaged blocks through it
+    std::vector<int32_t> leaf_base;
     // for the staged-pattern probe (the kernel itself needs neither): LDS byte offset of
     // every staged leaf in walk order (-1: gathered), first leaf of every stage ([K] = S)
     std::vector<int32_t> leaf_lds;

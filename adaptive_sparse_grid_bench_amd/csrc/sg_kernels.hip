@@ -860,19 +860,22 @@ int upload_tables(const Grid& g, DeviceTables& t) {
     if ((st = upload(t.dfs_base, g.dfs_base.data(), g.dfs_base.size())) != kOk) return st;
     if ((st = upload(t.dfs_bound, g.dfs_bound.data(), g.dfs_bound.size())) != kOk) return st;
     {
-        // staged-pattern probe tables from the walk's stage plans
-        std::vector<int32_t> lsum(size_t(g.S), 0);
-        for (int64_t q = 0; q < g.S; ++q) {
-            const auto it = std::lower_bound(g.sub_base.begin(), g.sub_base.end(), g.dfs_base[q]);
-            const int64_t id = it - g.sub_base.begin();
-            int sum = 0;
-            for (int j = 0; j < g.d; ++j) sum += g.sub_levels[size_t(id * g.d + j)];
-            lsum[size_t(q)] = sum;
-        }
+        // staged-pattern probe tables from the walks' stage plans (each plan lists its
+        // leaves in its own walk order)
         for (int kind = 0; kind < 2; ++kind)
             for (int e = 0; e < 2; ++e) {
                 const bool ok = kind == 0 ? g.stage[e].ok : g.blk[e].ok;
                 if (!ok) continue;
+                const std::vector<int32_t>& bases = kind == 0 ? g.dfs_base : g.blk[e].leaf_base;
+                std::vector<int32_t> lsum(size_t(g.S), 0);
+                for (int64_t q = 0; q < g.S; ++q) {
+                    const auto it =
+                        std::lower_bound(g.sub_base.begin(), g.sub_base.end(), bases[size_t(q)]);
+                    const int64_t id = it - g.sub_base.begin();
+                    int sum = 0;
+                    for (int j = 0; j < g.d; ++j) sum += g.sub_levels[size_t(id * g.d + j)];
+                    lsum[size_t(q)] = sum;
+                }
                 const std::vector<int32_t>& lds = kind == 0 ? g.stage[e].leaf_lds : g.blk[e].leaf_lds;
                 const std::vector<int32_t>& first =
                     kind == 0 ? g.stage[e].stage_first : g.blk[e].stage_first;
@@ -894,7 +897,7 @@ int upload_tables(const Grid& g, DeviceTables& t) {
                 }
                 for (int64_t q = 0; q < g.S; ++q)
                     if (lds[size_t(q)] < 0) {
-                        glob.push_back(g.dfs_base[size_t(q)]);
+                        glob.push_back(bases[size_t(q)]);
                         glob.push_back(lsum[size_t(q)]);
                     }
                 DeviceTables::Probe& pr = t.probe[kind][e];
@@ -935,10 +938,12 @@ int upload_tables(const Grid& g, DeviceTables& t) {
         if (!p.ok) continue;
         std::vector<int32_t> all(p.seg_begin);
         all.insert(all.end(), p.segs.begin(), p.segs.end());
+        all.insert(all.end(), p.leaf_base.begin(), p.leaf_base.end());
         if ((st = upload(t.blk_tabs[e], all.data(), all.size())) != kOk) return st;
         eval::GridDesc::BlkStages& sd = desc.blk[e];
         sd.seg_begin = t.blk_tabs[e];
         sd.segs = sd.seg_begin + p.seg_begin.size();
+        sd.leaf_base = sd.segs + p.segs.size();
         sd.num_stages = p.num_stages;
     }
     t.host_desc = desc;

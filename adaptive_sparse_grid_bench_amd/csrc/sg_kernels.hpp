@@ -40,6 +40,7 @@ struct GridDesc {
     struct BlkStages {
         const int32_t* seg_begin;
         const int32_t* segs;
+        const int32_t* leaf_base;   // subspace base offsets in this walk's leaf order
         int32_t num_stages;
         int32_t pad;
     } blk[2];
@@ -138,7 +139,7 @@ constexpr int kBlkVariant = 4;
 // d = 5..16 and 20 (the autotuner refines them per grid shape); a grid the block-staged
 // walk does not cover runs the staged walk
 inline int default_variant(int d, bool f32) {
-    if (f32) return d >= 8 && d <= 16 ? kBlkVariant : kStagedVariant;
+    if (f32) return d >= 8 ? kBlkVariant : kStagedVariant;
     return d >= 12 ? kBlkVariant : kStagedVariant;   // falls back to the staged walk
 }
 // the variant a launch runs: the staged walk falls back to variant 0 for a grid whose

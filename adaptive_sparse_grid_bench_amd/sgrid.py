@@ -65,6 +65,17 @@ class Tuning(C.Structure):
 EVAL_VARIANTS = 5
 STAGED_VARIANT = 3          # surplus blocks staged in LDS by the workgroup
 BLK_VARIANT = 4             # block-staged walk (restricted-regular grids, compile-time n)
+# the block-staged walk visits its blocks in the sparse order from these dimensions on
+# (sg_grid.hpp blk_sparse; tests/test_stage_plan.py checks the two agree): each leaf's
+# weight is the depth-first walk's, but the blocks are summed in another order, so the
+# outputs agree with the other variants within the FP64 bound, not bit for bit
+BLK_SPARSE_DMIN = {"f64": 12, "f32": 15}
+
+
+def blk_sparse(d: int, prec: str) -> bool:
+    """Whether the block-staged walk of a d-dimensional grid in precision prec ("f64" or
+    "f32") runs in the sparse order."""
+    return d >= BLK_SPARSE_DMIN[prec]
 
 
 class ServerConfig(C.Structure):

@@ -82,12 +82,14 @@ int check_plan(const sg::Grid& g, int e) {
     return 0;
 }
 
-// the static blocks of a regular grid's walk, in walk order: outer levels l_{d-1..J}
+// the static blocks of a regular grid's walk, in the kernel's visiting order: outer
+// levels l_{d-1..J}
 struct BlkWalk {
     const sg::Grid& g;
     int J;
     std::vector<std::vector<int>> blocks;   // outer levels (index k for dims J..d-1)
     std::vector<int> lv;
+    // depth first: dimension d-1 outermost, levels ascending
     void go(int k, int rem) {
         if (k == J - 1) {
             blocks.push_back(lv);
@@ -98,6 +100,17 @@ struct BlkWalk {
             go(k - 1, rem - l);
         }
         lv[k] = 0;
+    }
+    // sparse: below a node with dimensions >= c fixed, the children l_k = 1..rem for
+    // k = c-1 down to J, then the node's own block (dimensions J..c-1 at level 0)
+    void sparse(int c, int rem) {
+        for (int k = c - 1; k >= J && rem > 0; --k)
+            for (int l = 1; l <= rem; ++l) {
+                lv[k] = l;
+                sparse(k, rem - l);
+                lv[k] = 0;
+            }
+        blocks.push_back(lv);
     }
 };
 
@@ -116,7 +129,8 @@ int check_blk_plan(const sg::Grid& g, int e) {
     std::vector<uint8_t> src(size_t(g.N) * elem);
     for (size_t i = 0; i < src.size(); ++i) src[i] = uint8_t((i * 2654435761u) >> 11);
     BlkWalk w{g, J, {}, std::vector<int>(g.d, 0)};
-    w.go(g.d - 1, nm1);
+    if (sg::blk_sparse(g.d, elem)) w.sparse(g.d, nm1);
+    else w.go(g.d - 1, nm1);
     // the kernel's stage assignment
     struct Leaf { int64_t base, bytes; int lds; };
     std::vector<std::vector<Leaf>> stage_leaves;
@@ -207,9 +221,10 @@ int check_blk_plan(const sg::Grid& g, int e) {
                     return fail("blk leaf bytes differ from its surpluses", g.d, elem, k);
         }
     }
-    std::printf("d=%d n=%d elem=%d block-staged: %d stages, %lld full + %lld partial segments, %lld of %lld leaves staged (%lld blocks by slices): ok\n",
-                g.d, g.n, elem, K, (long long)full, (long long)partial, (long long)staged,
-                (long long)g.S, (long long)sliced);
+    std::printf("d=%d n=%d elem=%d block-staged (%s order): %d stages, %lld full + %lld partial segments, %lld of %lld leaves staged (%lld blocks by slices): ok\n",
+                g.d, g.n, elem, sg::blk_sparse(g.d, elem) ? "sparse" : "depth-first", K,
+                (long long)full, (long long)partial, (long long)staged, (long long)g.S,
+                (long long)sliced);
     return 0;
 }
 

@@ -1,11 +1,16 @@
 """The block-staged walk (evaluate variant 4) and the staged-pattern probe.
 
-Variant 4 runs on regular grids with n - 1 in 2..7, FP64 for d = 11..20 and FP32 for
-d = 7..16 (other grids run variant 3).  Its static blocks sit in LDS in walk order at
-compile-time offsets (n - 1 is a template constant).  Bars:
-  * outputs bit-identical to variant 0 (same leaves, same order, same arithmetic per
-    point; the FP64 walk computes each outer hat from x at its level, which on the 2^-24
-    lattice equals the incremental form) and within the FP64 bound of the oracle,
+Variant 4 runs on regular grids with n - 1 in 2..7, FP64 for d = 9..20 and FP32 for
+d = 7..20 (other grids run variant 3).  Its static blocks sit in LDS in walk order at
+compile-time offsets (n - 1 is a template constant).  From d = 12 (FP64) and d = 15
+(FP32) on it walks the sparse order (sgrid.blk_sparse).  Bars:
+  * depth-first order: outputs bit-identical to variant 0 (same leaves, same order, same
+    arithmetic per point; the FP64 walk computes each outer hat from x at its level,
+    which on the 2^-24 lattice equals the incremental form);
+  * sparse order: the blocks are summed in another order and one-leaf blocks take a
+    tabled product of level-0 hats, so outputs are within the bound of variant 0's and
+    of the oracle's, not bit-identical;
+  * every output within the FP64 bound of the oracle,
     |gpu - oracle| <= 1e-12 * sum_j |term_j|; FP32 within 2e-6 sum_j |term_j| + 1e-7;
   * interpolation at every grid point of the d=10, n=8 (FP32) and d=20, n=6 (FP64) grids;
   * ragged point counts (idle lanes in the last workgroup);
@@ -46,9 +51,21 @@ def evaluate(torch, g, alpha, pts, dt, variant):
     return host(o)
 
 
-SHAPES64 = [(11, 3, 900), (11, 8, 300), (12, 5, 700), (14, 4, 800), (16, 6, 200),
-            (17, 4, 500), (20, 3, 1000), (20, 5, 300)]
-SHAPES32 = [(7, 8, 900), (8, 3, 1000), (10, 8, 500), (13, 6, 300), (16, 4, 700)]
+SHAPES64 = [(9, 6, 700), (11, 3, 900), (11, 8, 300), (12, 5, 700), (14, 4, 800),
+            (16, 6, 200), (17, 4, 500), (20, 3, 1000), (20, 5, 300), (20, 8, 40)]
+SHAPES32 = [(7, 8, 900), (8, 3, 1000), (10, 8, 500), (13, 6, 300), (15, 5, 400), (16, 4, 700),
+            (18, 5, 300), (20, 6, 100)]
+
+
+def same_walk_result(o4, o0, d, prec, absref):
+    """Variant 4's outputs against variant 0's: bit-identical in the depth-first order,
+    within the precision's bound (relative to sum |terms|) in the sparse order."""
+    from adaptive_sparse_grid_bench_amd import sgrid
+    if not sgrid.blk_sparse(d, prec):
+        return np.array_equal(o4, o0)
+    e = np.abs(o4.astype(np.float64) - o0.astype(np.float64))
+    bound = 2e-12 * absref + 1e-300 if prec == "f64" else 4e-6 * absref + 2e-7
+    return bool((e <= bound).all())
 
 
 @pytest.mark.parametrize("prec,d,n,P", [("f64",) + s for s in SHAPES64] +
@@ -68,7 +85,7 @@ def test_blk_equals_variant0_and_oracle(sg, torch_gpu, prec, d, n, P):
     dt = torch.float64 if prec == "f64" else torch.float32
     o4 = evaluate(torch, g, alpha, pts, dt, BLK)
     o0 = evaluate(torch, g, alpha, pts, dt, 0)
-    assert np.array_equal(o4, o0)
+    assert same_walk_result(o4, o0, d, prec, absref)
     err = np.abs(o4.astype(np.float64) - ref)
     if prec == "f64":
         assert (err <= 1e-12 * absref + 1e-300).all(), float((err / (absref + 1e-300)).max())
@@ -84,10 +101,11 @@ def test_blk_ragged_counts(sg, torch_gpu):
     g = sg.Grid(d, n)
     for P in (0, 1, 63, 255, 256, 257, 511, 513):
         pts = W.uniform_points(max(P, 1), d, seed=P)[:P]
-        for dt in (torch.float64, torch.float32):
+        absref = O.evaluate(sh, np.abs(alpha), pts) if P else np.zeros(0)
+        for dt, prec in ((torch.float64, "f64"), (torch.float32, "f32")):
             o4 = evaluate(torch, g, alpha, pts, dt, BLK)
             o0 = evaluate(torch, g, alpha, pts, dt, 0)
-            assert np.array_equal(o4, o0), (P, dt)
+            assert same_walk_result(o4, o0, d, prec, absref), (P, dt)
 
 
 @pytest.mark.parametrize("d,n,prec", [(10, 8, "f32"), (20, 6, "f64")])

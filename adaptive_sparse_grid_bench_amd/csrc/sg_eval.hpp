@@ -701,9 +701,12 @@ struct LaneBlk {
 // d = 10, no spills; unrolling any level above the blocks made the compiler spill).  FP32
 // has registers to spare and takes the general walk with its unrolled levels and
 // incremental hats (d = 10, n = 8, 2^24 points: 95.8 ms against 133 ms for the lean walk).
+#ifndef SG_BLK_LEAN64
+#define SG_BLK_LEAN64 1
+#endif
 template <class T>
 constexpr bool blk_lean_walk() {
-    return sizeof(T) == 8;
+    return sizeof(T) == 8 && SG_BLK_LEAN64;
 }
 // Dimension J's levels 0..rem above the static blocks, as one switch that falls through:
 // entering at case rem runs the static blocks of budgets rem, rem - 1, ..., 0 (level

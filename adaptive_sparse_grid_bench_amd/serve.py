@@ -111,9 +111,15 @@ def _percentiles(lat):
     return float(np.percentile(a, 50)), float(np.percentile(a, 99))
 
 
-def run_server(grids, reqs, precision: str, arrivals=None, slots: int = 3,
-               max_batch_points: int = 1 << 20):
-    """Serve ``reqs`` (all at once if ``arrivals`` is None, else at those offsets in s)."""
+def run_server(grids, reqs, precision: str, arrivals=None, slots: int = 4,
+               max_batch_points: int = 1 << 19):
+    """Serve ``reqs`` (all at once if ``arrivals`` is None, else at those offsets in s).
+
+    Four slots of at most 2^19 points per launch: with 3 slots of 2^20 points the
+    closed-loop capacity is higher (43-46 against 34 M points/s FP32-Kahan) but Poisson
+    arrivals at 0.8 of it ran either 31 or 23 M points/s (p50 72 or 228 ms) from one run
+    to the next, as single launches of up to 10^6 points held the slots; with 4 x 2^19
+    two runs agree within 5 % (p50 49-52 ms)."""
     srv = sgrid.Server(precision, slots=slots, max_batch_points=max_batch_points,
                        max_requests=2048)
     key = "a64" if precision == "f64" else "a32"

@@ -313,6 +313,9 @@ def main() -> int:
     ap.add_argument("--probe-only", action="store_true",
                     help="only run the headline evaluate and its staged-pattern probe "
                          "(for profiling both kernels)")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="only run the precision form (d=20, n=6, FP64 and FP32-Kahan "
+                         "evaluation of 2^log2-points points; for profiling)")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -331,6 +334,13 @@ def main() -> int:
               "(evaluation would shard by query points with the grid replicated).",
               file=sys.stderr)
         return 2
+
+    if args.precision_only:
+        import torch
+        torch.cuda.set_device(0)
+        res, _ = precision_form(args, torch.cuda.current_stream())
+        print(json.dumps({"precision_only": True, **res}), flush=True)
+        return 0
 
     import torch
     from adaptive_sparse_grid_bench_amd import sgrid

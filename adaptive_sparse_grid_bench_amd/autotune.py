@@ -2,8 +2,10 @@
 
 For a grid it times, on the GPU and with HIP events on the launching stream:
   * the evaluate kernel over the compiled walk variants (tabled dimensions, gathers per
-    batch, waves per SIMD, the staged and block-staged walks), the workgroup size and
-    spatial binning on or off, in FP64 and FP32 (Kahan);
+    batch, waves per SIMD, the staged and block-staged walks, one or two points per
+    lane), the workgroup size and spatial binning on or off, in FP64 and FP32 (Kahan);
+    a variant the grid does not cover runs another one (``Grid.walk_variant``) and is
+    recorded as such instead of being timed again;
   * the hierarchization over the pole-family tile size (values per workgroup).
 The fastest choice is applied with ``Grid.set_tuning`` and stored in a JSON cache keyed
 by the grid shape (d, n, restriction vector or a digest of the explicit level set), so a
@@ -89,13 +91,19 @@ def tune(g: "sgrid.Grid", points: int = 1 << 17, reps: int = 3, precisions=("f64
                                                             device=dev))}
     base = g.get_tuning()
     timings = {}
+    runs_as = {}
     best = dict(base)
     for prec in precisions:
         a, x, o = data[prec]
         key = "eval_variant_f64" if prec == "f64" else "eval_variant_f32"
         cand = []
         for variant in range(sgrid.EVAL_VARIANTS):
-            staged = variant in (sgrid.STAGED_VARIANT, sgrid.BLK_VARIANT)
+            # a variant this grid does not cover runs another one: time each walk once
+            walk = g.walk_variant(variant, prec)
+            if walk != variant:
+                runs_as[f"{prec}/v{variant}"] = walk
+                continue
+            staged = variant in sgrid.STAGED_VARIANTS
             # the staged walks choose their workgroup size themselves and never bin
             for block in ((256,) if staged else EVAL_BLOCKS):
                 for binning in ((-1,) if staged else (-1, 4096)):
@@ -123,7 +131,7 @@ def tune(g: "sgrid.Grid", points: int = 1 << 17, reps: int = 3, precisions=("f64
     hcand.sort()
     best["hier_tile_values"] = hcand[0][1]
     g.set_tuning(**best)
-    record = {"tuning": best, "timings_ms": timings, "points": points,
+    record = {"tuning": best, "timings_ms": timings, "runs_as": runs_as, "points": points,
               "tuned_in_s": round(time.time() - t_start, 2),
               "library_version": sgrid.lib.sg_version()}
     if cache_path:

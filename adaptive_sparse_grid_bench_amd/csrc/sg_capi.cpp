@@ -98,7 +98,7 @@ int evaluate_binned(const sg_grid* h, const T* a, const T* x, int64_t P, T* o, v
     const int v = sg::effective_variant(h->g, sizeof(T) == 8 ? h->tune.variant : h->tune32.variant,
                                         int(sizeof(T)));
     if (P < h->tune.bin_min_points || h->tune.bin_min_points <= 0 || h->g.S < 1024 ||
-        v == sg::kStagedVariant || v == sg::kBlkVariant)
+        v == sg::kStagedVariant || v == sg::kBlkVariant || v == sg::kBlk2Variant)
         return eval_one(h, a, x, P, o, nullptr, s);
     const int kb = sg::key_bits_for(P, h->g.d);
     const size_t ws_bytes = sg::sort_workspace_bytes(P, kb);
@@ -145,7 +145,7 @@ extern "C" {
 
 const char* sg_last_error(void) { return sg::last_error().c_str(); }
 const char* sg_status_string(int status) { return sg::status_name(status); }
-int sg_version(void) { return 202; }
+int sg_version(void) { return 204; }
 
 int sg_grid_create(int num_dims, int num_levels, const int* limits, sg_grid** out) {
     if (!out) return fail(sg::kInvalid, "sg_grid_create: out is NULL");
@@ -244,6 +244,16 @@ int sg_grid_set_tuning(sg_grid* grid, const sg_tuning* tu) {
             grid->uploaded = false;
         }
     }
+    return sg::kOk;
+}
+
+int sg_grid_walk_variant(const sg_grid* grid, int elem_bytes, int variant, int* walk) {
+    if (!grid || !walk) return fail(sg::kInvalid, "sg_grid_walk_variant: NULL argument");
+    if (elem_bytes != 8 && elem_bytes != 4)
+        return fail(sg::kInvalid, "sg_grid_walk_variant: elem_bytes must be 8 or 4");
+    if (variant < 0 || variant >= SG_EVAL_VARIANTS)
+        return fail(sg::kInvalid, "sg_grid_walk_variant: unknown evaluate variant");
+    *walk = sg::effective_variant(grid->g, variant, elem_bytes);
     return sg::kOk;
 }
 

@@ -760,6 +760,193 @@ struct WalkerBlk {
     }
 };
 
+// ------------------------------------------------------ two points per lane --
+// The block-staged walk with two query points per lane (variant 5): a workgroup of 256
+// lanes evaluates 512 points.  Both points of a lane visit the same leaves in the same
+// order, so the stage copies, the stage barriers and the walk's scalar work (loop control,
+// stage checks) are shared by twice the points, while every per-point operation (hats,
+// addresses, reads, sums) is the single-point walk's, done once per point.  Each point's
+// arithmetic is exactly the lean walk's, so the outputs are bit-identical to variant 4's
+// (and variant 0's).
+
+// the two points' accumulators; block_begin pins both
+template <class Acc>
+struct AccPair {
+    using Inner = Acc;
+    Acc a[2];
+    __device__ __forceinline__ void pin() {
+        a[0].pin();
+        a[1].pin();
+    }
+};
+
+template <int D, int J, class T>
+struct LaneBlk2 {
+    LaneBlk<D, J, T> p[2];
+};
+
+template <int J, int R, int B, int c0, int cend, class Ln2, class T, class Acc>
+__device__ __forceinline__ void consume2(const T (&a0)[B], const T (&a1)[B], T (&sn0)[J],
+                                         T (&sn1)[J], const Ln2& ln, T pp0, T pp1,
+                                         AccPair<Acc>& acc) {
+    if constexpr (kNestedSum<Acc>) {
+        block_consume_nested<J, R, B, c0, cend>(a0, sn0, ln.p[0], pp0, acc.a[0]);
+        block_consume_nested<J, R, B, c0, cend>(a1, sn1, ln.p[1], pp1, acc.a[1]);
+    } else {
+        block_consume<J, R, B, c0, cend>(a0, ln.p[0], pp0, acc.a[0]);
+        block_consume<J, R, B, c0, cend>(a1, ln.p[1], pp1, acc.a[1]);
+    }
+}
+
+// a whole staged block at LDS offsets vb0 / vb1 (block base + each point's in-block offset)
+template <int J, int R, int H, int B, int c0, class Ln2, class T, class St>
+__device__ __forceinline__ void block_batches_blk2(const Ln2& ln, T pp0, T pp1, uint32_t vb0,
+                                                   uint32_t vb1, St& st, T (&sn0)[J],
+                                                   T (&sn1)[J]) {
+    constexpr int C = Simplex<J, R>::C;
+    if constexpr (c0 < C) {
+        constexpr int cend = (c0 + B < C) ? c0 + B : C;
+        T a0[B], a1[B];
+        block_issue_blk<J, R, H, B, c0, cend>(a0, ln.p[0], vb0, st);
+        block_issue_blk<J, R, H, B, c0, cend>(a1, ln.p[1], vb1, st);
+        consume2<J, R, B, c0, cend>(a0, a1, sn0, sn1, ln, pp0, pp1, st.acc);
+        block_batches_blk2<J, R, H, B, cend>(ln, pp0, pp1, vb0, vb1, st, sn0, sn1);
+    }
+}
+
+template <int J, int R, int H, int B, int c0, int c1, class Ln2, class T, class St>
+__device__ __forceinline__ void slice_batches2(const Ln2& ln, T pp0, T pp1, uint32_t vb0,
+                                               uint32_t vb1, St& st, T (&sn0)[J], T (&sn1)[J]) {
+    if constexpr (c0 < c1) {
+        constexpr int cend = (c0 + B < c1) ? c0 + B : c1;
+        T a0[B], a1[B];
+        slice_issue<J, R, H, B, c0, cend>(a0, ln.p[0], vb0, st);
+        slice_issue<J, R, H, B, c0, cend>(a1, ln.p[1], vb1, st);
+        consume2<J, R, B, c0, cend>(a0, a1, sn0, sn1, ln, pp0, pp1, st.acc);
+        __builtin_amdgcn_sched_barrier(0);
+        slice_batches2<J, R, H, B, cend, c1>(ln, pp0, pp1, vb0, vb1, st, sn0, sn1);
+    }
+}
+
+// a block too large for a stage, slice by slice (block_slices)
+template <int J, int R, int H, int B, int a, class Ln2, class T, class St>
+__device__ __forceinline__ void block_slices2(const Ln2& ln, T pp0, T pp1, uint32_t off0,
+                                              uint32_t off1, St& st, T (&sn0)[J], T (&sn1)[J]) {
+    if constexpr (a <= R) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            asm volatile("" : "+v"(sn0[j]));
+            asm volatile("" : "+v"(sn1[j]));
+        }
+        st.template block_begin<uint32_t(blk_bytes(J - 1, R - a, H + a, int(sizeof(T))))>();
+        launder_offsets<J, R>(const_cast<Ln2&>(ln).p[0]);
+        launder_offsets<J, R>(const_cast<Ln2&>(ln).p[1]);
+        slice_batches2<J, R, H, B, slice_first<J, R>(a), slice_first<J, R>(a + 1)>(
+            ln, pp0, pp1, off0 + st.base, off1 + st.base, st, sn0, sn1);
+        block_slices2<J, R, H, B, a + 1>(ln, pp0, pp1, off0, off1, st, sn0, sn1);
+    }
+}
+
+// a block gathered from global memory
+template <int J, int R, int B, int c0, class Ln2, class T, class St>
+__device__ __forceinline__ void block_batches2_global(const Ln2& ln, T pp0, T pp1, uint32_t off0,
+                                                      uint32_t off1, int hsum, St& st,
+                                                      T (&sn0)[J], T (&sn1)[J]) {
+    constexpr int C = Simplex<J, R>::C;
+    if constexpr (c0 < C) {
+        constexpr int cend = (c0 + B < C) ? c0 + B : C;
+        T a0[B], a1[B];
+        block_issue<J, R, B, false, c0, cend>(a0, ln.p[0], off0, hsum, st);
+        block_issue<J, R, B, false, c0, cend>(a1, ln.p[1], off1, hsum, st);
+        consume2<J, R, B, c0, cend>(a0, a1, sn0, sn1, ln, pp0, pp1, st.acc);
+        block_batches2_global<J, R, B, cend>(ln, pp0, pp1, off0, off1, hsum, st, sn0, sn1);
+    }
+}
+
+template <int J, int R, int B, class Ln2, class T, class St>
+__device__ __forceinline__ void static_block2(const Ln2& ln, T pp0, T pp1, uint32_t off0,
+                                              uint32_t off1, int hsum, St& st) {
+    static_assert(St::kBlk, "two points per lane run the block-staged walk");
+    constexpr int H = St::kNM1 - R;
+    constexpr int E = int(sizeof(T));
+    constexpr int kHow = blk_staging(J, R, H, E, St::kStageBytes);
+    T sn0[J], sn1[J];
+    if constexpr (kHow == 1) {
+        st.template block_begin<uint32_t(blk_bytes(J, R, H, E))>();
+        launder_offsets<J, R>(const_cast<Ln2&>(ln).p[0]);
+        launder_offsets<J, R>(const_cast<Ln2&>(ln).p[1]);
+        block_batches_blk2<J, R, H, B, 0>(ln, pp0, pp1, off0 + st.base, off1 + st.base, st, sn0,
+                                          sn1);
+    } else if constexpr (kHow == 2) {
+        block_slices2<J, R, H, B, 0>(ln, pp0, pp1, off0, off1, st, sn0, sn1);
+    } else {
+        block_batches2_global<J, R, B, 0>(ln, pp0, pp1, off0, off1, hsum, st, sn0, sn1);
+    }
+    st.q += Simplex<J, R>::C;
+}
+
+template <int J, int LT, int B, class Ln2, class T, class St>
+__device__ __forceinline__ void static_block_dispatch2(int R, const Ln2& ln, T pp0, T pp1,
+                                                       uint32_t off0, uint32_t off1, int hsum,
+                                                       St& st) {
+    switch (R) {
+#define SG_BLK2(RR) \
+    case RR: if constexpr (RR < LT) static_block2<J, RR, B>(ln, pp0, pp1, off0, off1, hsum, st); break;
+        SG_BLK2(0) SG_BLK2(1) SG_BLK2(2) SG_BLK2(3) SG_BLK2(4) SG_BLK2(5) SG_BLK2(6) SG_BLK2(7)
+#undef SG_BLK2
+        default: break;
+    }
+}
+
+template <int J, int B, int R, class Ln2, class T, class St>
+__device__ __forceinline__ void blk_level_run2(Ln2& ln, T pp0, T pp1, uint32_t off0,
+                                               uint32_t off1, int hsum, int rem, St& st) {
+    T phi0, phi1;
+    uint32_t t0, t1;
+    const int l = rem - R;
+    hat_at(ln.p[0].x[J], l, phi0, t0);
+    hat_at(ln.p[1].x[J], l, phi1, t1);
+    static_block2<J, R, B>(ln, pp0 * phi0, pp1 * phi1,
+                           off0 + ((t0 * uint32_t(sizeof(T))) << hsum),
+                           off1 + ((t1 * uint32_t(sizeof(T))) << hsum), hsum + l, st);
+}
+
+template <int J, int B, class Ln2, class T, class St>
+__device__ __forceinline__ void blk_levels2(Ln2& ln, T pp0, T pp1, uint32_t off0, uint32_t off1,
+                                            int hsum, int rem, St& st) {
+    switch (rem) {
+#define SG_LV2(RR) \
+    case RR: blk_level_run2<J, B, RR>(ln, pp0, pp1, off0, off1, hsum, rem, st); [[fallthrough]];
+        SG_LV2(7) SG_LV2(6) SG_LV2(5) SG_LV2(4) SG_LV2(3) SG_LV2(2) SG_LV2(1) SG_LV2(0)
+#undef SG_LV2
+        default: break;
+    }
+}
+
+// the lean block-staged walk (WalkerBlk) for both points of a lane
+template <int K, int D, int J, int B, class T, class St>
+struct WalkerBlk2 {
+    using Ln2 = LaneBlk2<D, J, T>;
+    static __device__ __forceinline__ void run(Ln2& ln, T pp0, T pp1, uint32_t off0,
+                                               uint32_t off1, int hsum, int rem, St& st) {
+        if constexpr (K == J - 1) {
+            static_block_dispatch2<J, 8, B>(rem, ln, pp0, pp1, off0, off1, hsum, st);
+        } else if constexpr (K == J && D <= SG_BLK_FALL_DMAX) {
+            blk_levels2<J, B>(ln, pp0, pp1, off0, off1, hsum, rem, st);
+        } else {
+            for (int l = 0; l <= rem; ++l) {
+                T phi0, phi1;
+                uint32_t t0, t1;
+                hat_at(ln.p[0].x[K], l, phi0, t0);
+                hat_at(ln.p[1].x[K], l, phi1, t1);
+                WalkerBlk2<K - 1, D, J, B, T, St>::run(
+                    ln, pp0 * phi0, pp1 * phi1, off0 + ((t0 * uint32_t(sizeof(T))) << hsum),
+                    off1 + ((t1 * uint32_t(sizeof(T))) << hsum), hsum + l, rem - l, st);
+            }
+        }
+    }
+};
+
 // Lane state of the sparse block-staged walk: the J tabled dimensions' hat / offset
 // tables.  The coordinates of the outer dimensions J..D-1, which the walk indexes with a
 // wave-uniform dimension known only at run time, live in separate kernel-local arrays of
@@ -1057,6 +1244,62 @@ void k_eval(Params<T> prm) {
     if (live) prm.out[G->out_off + pc] = st.acc.value();
 }
 
+// The block-staged lean walk with two points per lane (variant 5; one-grid launches of a
+// regular grid with n - 1 = NM1): workgroup w evaluates points w 2 TPB + i TPB + lane,
+// i = 0, 1 (coalesced coordinate loads per half).  Idle points walk the workgroup's first
+// point and store nothing.
+template <int D, int J, int B, int W, class T, class Acc, int NM1, int TPB, int STB>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(W)))
+void k_eval2(Params<T> prm) {
+    static_assert(NM1 >= 0 && J > 0 && J < D, "a regular block-staged walk");
+    const cptr<GridDesc> G = as_const(prm.grids);
+    const int64_t first = int64_t(blockIdx.x) * (2 * TPB);
+    LaneBlk2<D, J, T> ln;
+    int64_t pc[2];
+    bool live[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int64_t p = first + i * TPB + threadIdx.x;
+        live[i] = p < prm.P;
+        pc[i] = live[i] ? (prm.perm ? int64_t(prm.perm[p]) : p) : first;
+        const T* x = prm.coords + G->coord_off + pc[i] * D;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const T xk = x[k];
+            if (k < J) level_table<8, T>(xk, ln.p[i].ph[k < J ? k : 0], ln.p[i].ub[k < J ? k : 0]);
+            ln.p[i].x[k] = xk;
+        }
+    }
+    const T* alpha = prm.alpha ? prm.alpha + G->alpha_off : static_cast<const T*>(G->alpha_ptr);
+    using St = StateBlk<T, AccPair<Acc>, NM1, TPB, STB>;
+    St st{};
+    st.alpha = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(alpha), short(0),
+                                                 int(G->N * int64_t(sizeof(T))), 0x00020000);
+    st.bounds = as_const(G->bounds);
+    st.q = 0;
+    st.bq = 0;
+    __shared__ __attribute__((aligned(16))) uint8_t blk_buf[2 * STB];
+    const cptr<GridDesc::BlkStages> tb = &G->blk[sizeof(T) == 8 ? 0 : 1];
+    st.lds = (lds_byte*)blk_buf;
+    st.seg_begin = as_const(tb->seg_begin);
+    st.segs = as_const(tb->segs);
+    st.leaf_base = as_const(tb->leaf_base);
+    st.num_stages = tb->num_stages;
+    st.stage = 0;
+    st.base = 0;
+    st.cur = 0;
+    st.limit = uint32_t(STB);
+    if (st.num_stages > 0) {
+        st.issue(0);
+        stage_barrier();
+        if (st.num_stages > 1) st.issue(1);
+    }
+    WalkerBlk2<D - 1, D, J, B, T, St>::run(ln, T(1), T(1), 0u, 0u, 0, NM1, st);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        if (live[i]) prm.out[G->out_off + pc[i]] = st.acc.a[i].value();
+}
+
 // The staged-pattern probe: the data path of a staged walk without its arithmetic.  The
 // workgroup runs the walk's stage plan (the same buffer-to-LDS copies and stage barriers
 // between them); in stage k every lane reads as many coefficients from LDS as the walk
@@ -1079,7 +1322,8 @@ struct ProbeParams {
     T* out;
 };
 
-template <class T, int TPB, int STB>
+// PPL points per lane (the two-point walk's data path: each stage serves 2 TPB points)
+template <class T, int TPB, int STB, int PPL = 1>
 __global__ __launch_bounds__(TPB) void k_stage_probe(ProbeParams<T> p) {
     __shared__ __attribute__((aligned(16))) uint8_t buf[2 * STB];
     lds_byte* lds = (lds_byte*)buf;
@@ -1087,13 +1331,20 @@ __global__ __launch_bounds__(TPB) void k_stage_probe(ProbeParams<T> p) {
         const_cast<T*>(p.alpha), short(0), int(p.N * int64_t(sizeof(T))), 0x00020000);
     const cptr<int32_t> info = as_const(p.info), glob = as_const(p.glob),
                         seg_begin = as_const(p.seg_begin), segs = as_const(p.segs);
-    const int64_t pi = int64_t(blockIdx.x) * TPB + threadIdx.x;
-    uint32_t h = uint32_t(pi) * 0x9E3779B9u ^ p.seed;
-    h ^= h >> 15;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    const T w = T(1) + T(h & 1023u) * T(1.0 / 4096);
-    T acc = T(0);
+    int64_t pi[PPL];
+    uint32_t h[PPL];
+    T w[PPL], acc[PPL];
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) {
+        pi[r] = int64_t(blockIdx.x) * (PPL * TPB) + r * TPB + threadIdx.x;
+        uint32_t hr = uint32_t(pi[r]) * 0x9E3779B9u ^ p.seed;
+        hr ^= hr >> 15;
+        hr *= 0x85EBCA6Bu;
+        hr ^= hr >> 13;
+        h[r] = hr;
+        w[r] = T(1) + T(hr & 1023u) * T(1.0 / 4096);
+        acc[r] = T(0);
+    }
     const int K = p.num_stages;
     if (K > 0) {
         copy_stage<TPB>(rsrc, lds, seg_begin, segs, 0);
@@ -1108,31 +1359,38 @@ __global__ __launch_bounds__(TPB) void k_stage_probe(ProbeParams<T> p) {
         const int cnt = info[2 * k];
         const uint32_t mask = uint32_t(info[2 * k + 1]);
         const uint32_t base = uint32_t(k & 1) * uint32_t(STB);
-        uint32_t at[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            at[j] = base + (((h >> j) ^ (h >> (j + 11))) & mask) * uint32_t(sizeof(T));
-        int i = 0;
-        for (; i + 8 <= cnt; i += 8) {
-            asm volatile("" ::: "memory");     // re-read every pass (no load hoisting)
-            T v[8];
+        for (int r = 0; r < PPL; ++r) {
+            uint32_t at[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                v[j] = *reinterpret_cast<const __attribute__((address_space(3))) T*>(lds + at[j]);
+                at[j] = base + (((h[r] >> j) ^ (h[r] >> (j + 11))) & mask) * uint32_t(sizeof(T));
+            int i = 0;
+            for (; i + 8 <= cnt; i += 8) {
+                asm volatile("" ::: "memory");     // re-read every pass (no load hoisting)
+                T v[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) acc = fma(v[j], w, acc);
-        }
-        for (; i < cnt; ++i) {
-            asm volatile("" ::: "memory");
-            acc = fma(*reinterpret_cast<const __attribute__((address_space(3))) T*>(lds + at[i & 7]),
-                      w, acc);
+                for (int j = 0; j < 8; ++j)
+                    v[j] = *reinterpret_cast<const __attribute__((address_space(3))) T*>(lds + at[j]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) acc[r] = fma(v[j], w[r], acc[r]);
+            }
+            for (; i < cnt; ++i) {
+                asm volatile("" ::: "memory");
+                acc[r] = fma(*reinterpret_cast<const __attribute__((address_space(3))) T*>(
+                                 lds + at[i & 7]),
+                             w[r], acc[r]);
+            }
         }
     }
-    for (int q = 0; q < p.num_glob; ++q) {
-        const uint32_t cell = (h >> (q & 7)) & ((1u << glob[2 * q + 1]) - 1u);
-        acc = fma(gather_leaf<T>(rsrc, glob[2 * q], cell * uint32_t(sizeof(T))), w, acc);
+#pragma unroll
+    for (int r = 0; r < PPL; ++r) {
+        for (int q = 0; q < p.num_glob; ++q) {
+            const uint32_t cell = (h[r] >> (q & 7)) & ((1u << glob[2 * q + 1]) - 1u);
+            acc[r] = fma(gather_leaf<T>(rsrc, glob[2 * q], cell * uint32_t(sizeof(T))), w[r], acc[r]);
+        }
+        if (pi[r] < p.P) p.out[pi[r]] = acc[r];
     }
-    if (pi < p.P) p.out[pi] = acc;
 }
 
 // Launch the staged-pattern probe of grid g's staged walk in this precision (the
@@ -1162,9 +1420,11 @@ static_assert(blk_waves(8) % (blk_threads(8) / 256 > 0 ? blk_threads(8) / 256 : 
               "whole workgroups per CU");
 constexpr Variant kVariants[] = {{3, 8, 3, false}, {2, 8, 3, false}, {3, 4, 4, false},
                                  {kStageJ, 0, 3, true},     // batch: stage_batch()
-                                 {kStageJ, 0, blk_waves(8), true}};   // waves: per precision
+                                 {kStageJ, 0, blk_waves(8), true},    // waves: per precision
+                                 {kStageJ, 0, 2, true}};    // two points per lane
 static_assert(kStagedVariant == 3 && kVariants[kStagedVariant].staged, "staged variant index");
 static_assert(kBlkVariant == 4 && kVariants[kBlkVariant].staged, "block-staged variant index");
+static_assert(kBlk2Variant == 5 && kVariants[kBlk2Variant].staged, "two-point variant index");
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // tabled dimensions the explicit-set kernels use (their node-code stream depends on it)
 constexpr int kExplicitJ = kExplicitTabDims;
@@ -1206,6 +1466,51 @@ hipError_t launch_variant(int variant, int lt, const Params<T>& prm, int64_t blo
 #else
 ;
 #endif
+
+// The block-staged lean walk with two points per lane (FP64, depth-first order), 256
+// threads and 512 points per workgroup, 2 waves per SIMD (its lane state for two points
+// needs the 256 VGPRs of that occupancy).
+constexpr int kBlk2Threads = 256;
+#ifndef SG_BLK2_BATCH
+#define SG_BLK2_BATCH 2
+#endif
+template <int D, class T, class Acc>
+hipError_t launch_blk2(int nm1, const Params<T>& prm, int64_t blocks, hipStream_t s)
+#ifdef SG_EVAL_INSTANTIATE
+{
+    constexpr int Jd = kStageJ < D ? kStageJ : D;
+    constexpr int E = int(sizeof(T));
+    if constexpr (blk2_built(D, E) && Jd < D) {
+        switch (nm1) {
+#define SG_NM2(NN)                                                                           \
+    case NN:                                                                                 \
+        hipLaunchKernelGGL((k_eval2<D, Jd, SG_BLK2_BATCH, 2, T, Acc, NN, kBlk2Threads,        \
+                                    blk_stage_bytes(E)>),                                    \
+                           dim3(unsigned(blocks)), dim3(kBlk2Threads), 0, s, prm);           \
+        return hipGetLastError();
+            SG_NM2(2) SG_NM2(3) SG_NM2(4) SG_NM2(5) SG_NM2(6) SG_NM2(7)
+#undef SG_NM2
+            default: break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+#else
+;
+#endif
+
+template <class T, class Acc>
+hipError_t launch_d_blk2(int d, int nm1, const Params<T>& prm, int64_t blocks, hipStream_t s) {
+    switch (d) {
+#define SG_D(DD) \
+    case DD: return launch_blk2<DD, T, Acc>(nm1, prm, blocks, s);
+        SG_D(1) SG_D(2) SG_D(3) SG_D(4) SG_D(5) SG_D(6) SG_D(7) SG_D(8) SG_D(9) SG_D(10)
+        SG_D(11) SG_D(12) SG_D(13) SG_D(14) SG_D(15) SG_D(16) SG_D(17) SG_D(18) SG_D(19)
+        SG_D(20)
+#undef SG_D
+        default: return hipErrorInvalidValue;
+    }
+}
 
 // The block-staged walk of a regular grid with n - 1 = nm1 (kBlkMinNM1 .. kBlkMaxNM1),
 // workgroups of blk_threads(sizeof(T)); built for the dimensions blk_built names.
@@ -1296,7 +1601,8 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
     const int v = effective_variant(g, tu.variant, int(sizeof(T)));
     const int block = v == kBlkVariant ? blk_threads(int(sizeof(T)))
                       : kVariants[v].staged ? kThreads : tu.block;
-    const int64_t blocks = (P + block - 1) / block;
+    const int points_per_block = v == kBlk2Variant ? 2 * kBlk2Threads : block;
+    const int64_t blocks = (P + points_per_block - 1) / points_per_block;
     if (blocks > int64_t(0x7fffffff)) {
         set_error("evaluate: too many points for one launch");
         return kRange;
@@ -1309,7 +1615,9 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
     prm.work = nullptr;
     prm.perm = perm;
     prm.P = P;
-    const hipError_t e = v == kBlkVariant
+    const hipError_t e = v == kBlk2Variant
+        ? launch_d_blk2<T, Acc>(g.d, g.n - 1, prm, blocks, s)
+        : v == kBlkVariant
         ? launch_d_blk<T, Acc>(g.d, g.n - 1, prm, blocks, s)
         : g.explicit_set
         ? launch_d<true, T, Acc>(g.d, v == kStagedVariant ? v : 0, table_levels(g, 0), prm,
@@ -1328,12 +1636,12 @@ int launch_stage_probe(const Grid& g, const DeviceTables& t, int variant, const 
     if (P == 0) return kOk;
     const int e = sizeof(T) == 8 ? 0 : 1;
     const int v = effective_variant(g, variant, int(sizeof(T)));
-    if (v != kBlkVariant && v != kStagedVariant) {
+    if (v != kBlkVariant && v != kBlk2Variant && v != kStagedVariant) {
         set_error("stage probe: the grid runs no staged walk in this precision");
         return kUnsupported;
     }
     const GridDesc& hd = t.host_desc;
-    const DeviceTables::Probe& pt = t.probe[v == kBlkVariant ? 1 : 0][e];
+    const DeviceTables::Probe& pt = t.probe[v == kStagedVariant ? 0 : 1][e];
     ProbeParams<T> p{};
     p.alpha = alpha;
     p.N = g.N;
@@ -1344,7 +1652,13 @@ int launch_stage_probe(const Grid& g, const DeviceTables& t, int variant, const 
     p.P = P;
     p.seed = seed;
     p.out = out;
-    if (v == kBlkVariant) {
+    if (v == kBlk2Variant) {
+        p.seg_begin = hd.blk[e].seg_begin;
+        p.segs = hd.blk[e].segs;
+        constexpr int TPB = kBlk2Threads;
+        hipLaunchKernelGGL((k_stage_probe<T, TPB, blk_stage_bytes(int(sizeof(T))), 2>),
+                           dim3(unsigned((P + 2 * TPB - 1) / (2 * TPB))), dim3(TPB), 0, s, p);
+    } else if (v == kBlkVariant) {
         p.seg_begin = hd.blk[e].seg_begin;
         p.segs = hd.blk[e].segs;
         constexpr int TPB = blk_threads(int(sizeof(T)));

@@ -20,7 +20,8 @@ using AA = AccKahan;
 // otherwise the other walks (launch_variant), so the heavy kernels build in parallel.
 #ifdef SG_BLK_ONLY
 #define SG_INST(DD) \
-    template hipError_t launch_blk<DD, TT, AA>(int, const Params<TT>&, int64_t, hipStream_t);
+    template hipError_t launch_blk<DD, TT, AA>(int, const Params<TT>&, int64_t, hipStream_t); \
+    template hipError_t launch_blk2<DD, TT, AA>(int, const Params<TT>&, int64_t, hipStream_t);
 #else
 #define SG_INST(DD)                                                                        \
     template hipError_t launch_variant<DD, false, TT, AA>(int, int, const Params<TT>&,     \

@@ -185,6 +185,18 @@ SG_HOST_DEVICE constexpr bool blk_sparse(int d, int elem) {
     return d >= (elem == 8 ? SG_BLK_SPARSE64_DMIN : SG_BLK_SPARSE32_DMIN);
 }
 
+// the two-points-per-lane kernels (variant 5): FP64, the dimensions with the block-staged
+// walk in the depth-first order
+//   Measured slower than one point per lane everywhere (2^22-2^24 points, ms, 2 / 1 points
+//   per lane): FP64 d = 9, n = 8: 21.6 / 18.7; d = 10, n = 8: 171.6 / 143.4 (2^24);
+//   d = 11, n = 7: 72.3 / 29.6; the same lean walk in FP32: d = 10: 137.9 / 99.3 (2^24).
+//   Two points' lane state needs the 256 VGPRs of 2 waves per SIMD and still spills
+//   (300 bytes per lane at d = 10); the halved stage copies (its probe: 90.3 -> 75.5 ms
+//   at d = 10) do not pay for that.  The autotuner keeps it as a candidate.
+SG_HOST_DEVICE constexpr bool blk2_built(int d, int elem) {
+    return elem == 8 && blk_built(d, elem) && !blk_sparse(d, elem);
+}
+
 // How the block-staged walk copies one grid's blocks into LDS, for one element size.
 struct BlockStagePlan {
     bool ok = false;

@@ -134,6 +134,8 @@ struct EvalTuning {
 constexpr int kStagedVariant = 3;
 // the block-staged walk (regular grids, n - 1 in kBlkMinNM1..kBlkMaxNM1)
 constexpr int kBlkVariant = 4;
+// the block-staged walk with two points per lane (blk2_built: FP64, depth-first order)
+constexpr int kBlk2Variant = 5;
 
 // default variants by dimension and precision, from timings at 2^22-2^24 points on
 // d = 5..16 and 20 (the autotuner refines them per grid shape); a grid the block-staged
@@ -146,6 +148,13 @@ inline int default_variant(int d, bool f32) {
 // blocks cannot be staged in this element size
 inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
     const int e = elem_bytes == 8 ? 0 : 1;
+    if (variant == kBlk2Variant) {
+        if (!blk2_built(g.d, elem_bytes)) variant = kBlkVariant;
+        else if (g.explicit_set || !g.blk[e].ok || g.n - 1 < kBlkMinNM1 || g.n - 1 > kBlkMaxNM1)
+            variant = kStagedVariant;
+        else
+            return variant;
+    }
     if (variant == kBlkVariant &&
         (g.explicit_set || !g.blk[e].ok || g.n - 1 < kBlkMinNM1 || g.n - 1 > kBlkMaxNM1 ||
          !blk_built(g.d, elem_bytes)))
@@ -157,7 +166,9 @@ inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
 // eval::kThreads points, the workgroup size of every walk there; a block-staged grid
 // whose workgroups differ in size runs the staged walk instead
 inline int work_list_variant(const Grid& g, int variant, int elem_bytes) {
-    const int v = effective_variant(g, variant, elem_bytes);
+    // work items hold one point per lane: the two-point walk runs as the block-staged one
+    const int v = effective_variant(g, variant == kBlk2Variant ? kBlkVariant : variant,
+                                    elem_bytes);
     if (v == kBlkVariant && blk_threads(elem_bytes) != 256)
         return effective_variant(g, kStagedVariant, elem_bytes);
     return v;

@@ -35,7 +35,7 @@ EXPORTS = [
     "sg_last_error", "sg_status_string", "sg_version",
     "sg_grid_create", "sg_grid_create_explicit", "sg_grid_destroy", "sg_grid_info",
     "sg_grid_tables", "sg_grid_subspaces", "sg_grid_get_tuning", "sg_grid_set_tuning",
-    "sg_gp2idx", "sg_idx2gp", "sg_idx2l", "sg_l2idx",
+    "sg_grid_walk_variant", "sg_gp2idx", "sg_idx2gp", "sg_idx2l", "sg_l2idx",
     "sg_idx2gp_dev", "sg_gp2idx_dev", "sg_grid_coords_f64",
     "sg_hierarchize_f64", "sg_hierarchize_f32",
     "sg_evaluate_f64", "sg_evaluate_f32c", "sg_evaluate_ref_f64", "sg_evaluate_ref_f32",
@@ -62,9 +62,11 @@ class Tuning(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-EVAL_VARIANTS = 5
+EVAL_VARIANTS = 6
 STAGED_VARIANT = 3          # surplus blocks staged in LDS by the workgroup
 BLK_VARIANT = 4             # block-staged walk (restricted-regular grids, compile-time n)
+BLK2_VARIANT = 5            # block-staged walk with two points per lane (FP64, depth-first)
+STAGED_VARIANTS = (STAGED_VARIANT, BLK_VARIANT, BLK2_VARIANT)
 # the block-staged walk visits its blocks in the sparse order from these dimensions on
 # (sg_grid.hpp blk_sparse; tests/test_stage_plan.py checks the two agree): each leaf's
 # weight is the depth-first walk's, but the blocks are summed in another order, so the
@@ -117,6 +119,7 @@ def _load() -> C.CDLL:
         "sg_grid_subspaces": (i, [vp, C.POINTER(C.c_ubyte), llp]),
         "sg_grid_get_tuning": (i, [vp, C.POINTER(Tuning)]),
         "sg_grid_set_tuning": (i, [vp, C.POINTER(Tuning)]),
+        "sg_grid_walk_variant": (i, [vp, i, i, ip]),
         "sg_gp2idx": (i, [vp, ip, ip, llp]),
         "sg_idx2gp": (i, [vp, ll, ip, ip]),
         "sg_idx2l": (i, [vp, ll, i, ip]),
@@ -320,6 +323,15 @@ class Grid:
             raise TypeError(f"unknown tuning fields {sorted(unknown)}")
         cur.update(kw)
         check(lib.sg_grid_set_tuning(self._h, C.byref(Tuning(**cur))))
+
+    def walk_variant(self, variant: int, prec: str = "f64") -> int:
+        """The walk variant an evaluate in precision prec ("f64" or "f32") runs on this
+        grid when its tuning names `variant` (a variant the grid does not cover falls
+        back to a more general one)."""
+        out = C.c_int()
+        check(lib.sg_grid_walk_variant(self._h, 8 if prec == "f64" else 4, variant,
+                                       C.byref(out)))
+        return out.value
 
     # ---- host bijection (asg.c:187-279) --------------------------------------
     def gp2idx(self, levels, indices) -> int:

@@ -40,7 +40,8 @@ FP64_PEAK_TFLOPS = 78.6
 VARIANT_NAMES = {0: "gather, 3 tabled dims, batches of 8", 1: "gather, 2 tabled dims",
                  2: "gather, batches of 4, 4 waves/SIMD",
                  3: "staged in LDS, 3 tabled dims, leaves at table-given LDS offsets",
-                 4: "block-staged in LDS, compile-time n, leaves at compile-time LDS offsets"}
+                 4: "block-staged in LDS, compile-time n, leaves at compile-time LDS offsets",
+                 5: "block-staged in LDS, two points per lane"}
 SAMPLE = 4096          # points of each timed output checked against the CPU oracle
 
 

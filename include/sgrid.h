@@ -86,12 +86,16 @@ int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *bas
  *                     dimensions, gathers per batch and waves per SIMD); 3 is the staged
  *                     walk, where the workgroup copies surplus blocks into LDS and the
  *                     lanes read them there (blocks too large for a 26 KiB stage are
- *                     gathered as in 0..2).  A grid whose three tabled dimensions reach
- *                     level 8 runs variant 0 instead of 3;
+ *                     gathered as in 0..2); 4 is the block-staged walk of regular grids
+ *                     (blocks at compile-time LDS offsets; a sparse block order from
+ *                     d = 12 in FP64 and d = 15 in FP32); 5 is the block-staged walk with
+ *                     two query points per lane (FP64 below d = 12; elsewhere it runs as
+ *                     4).  A grid a variant does not cover runs the next more general
+ *                     one (5 -> 4 -> 3 -> 0);
  *   bin_min_points    evaluate launches with at least this many points bin the points
- *                     spatially first (<= 0: never; the staged variant never bins);
+ *                     spatially first (<= 0: never; the staged variants never bin);
  *   hier_tile_values  target values per hierarchization workgroup tile (>= 64). */
-#define SG_EVAL_VARIANTS 5
+#define SG_EVAL_VARIANTS 6
 typedef struct sg_tuning {
     int eval_block;
     int eval_variant_f64;
@@ -101,6 +105,12 @@ typedef struct sg_tuning {
 } sg_tuning;
 int sg_grid_get_tuning(const sg_grid *grid, sg_tuning *tuning);
 int sg_grid_set_tuning(sg_grid *grid, const sg_tuning *tuning);
+/* The walk variant a single-grid evaluate of this grid runs when its tuning names
+ * `variant` (elem_bytes 8: sg_evaluate_f64, 4: sg_evaluate_f32c), after the fallback
+ * above: *walk = variant where the grid is covered, else the more general one that runs.
+ * Host only.  Lets a tuner time each distinct walk once.  No counterpart in the
+ * reference (it has one kernel per variant name). */
+int sg_grid_walk_variant(const sg_grid *grid, int elem_bytes, int variant, int *walk);
 
 /* ------------------------------------------------------ host bijection ------- */
 

@@ -86,6 +86,32 @@ def test_noalloc_evaluate_argument_checks():
     assert "SG_NOALLOC_MAX_POINTS" in sgrid.lib.sg_last_error().decode()
 
 
+def test_walk_variant_fallbacks():
+    """sg_grid_walk_variant reports the walk a tuned variant runs: the two-point walk
+    only in FP64 at d = 9..11 (depth-first order), the block-staged walks only on regular
+    grids with n - 1 in 2..7 (FP64 from d = 9, FP32 from d = 7), the staged walk on
+    explicit sets and where its stages hold the blocks, gathers elsewhere."""
+    from adaptive_sparse_grid_bench_amd import sgrid
+    g = sgrid.Grid(10, 8)
+    assert [g.walk_variant(v, "f64") for v in range(6)] == [0, 1, 2, 3, 4, 5]
+    assert [g.walk_variant(v, "f32") for v in range(6)] == [0, 1, 2, 3, 4, 4]
+    g20 = sgrid.Grid(20, 6)                  # sparse block order: one point per lane
+    assert g20.walk_variant(5, "f64") == 4 and g20.walk_variant(5, "f32") == 4
+    assert sgrid.Grid(12, 7).walk_variant(5, "f64") == 4
+    g8 = sgrid.Grid(8, 5)
+    assert g8.walk_variant(5, "f64") == 3 and g8.walk_variant(5, "f32") == 4
+    assert sgrid.Grid(6, 5).walk_variant(4, "f32") == 3
+    # n - 1 = 8: no block-staged kernel, and level-8 blocks exceed the staged walk's tables
+    assert [sgrid.Grid(10, 9).walk_variant(v, "f64") for v in (3, 4, 5)] == [0, 0, 0]
+    lv, _ = sgrid.Grid(4, 5, [4, 1, 4, 2]).subspaces()
+    ge = sgrid.Grid.explicit(4, lv)
+    assert [ge.walk_variant(v, "f64") for v in (3, 4, 5)] == [3, 3, 3]
+    for bad in ((g._h, 2, 0), (g._h, 8, 6), (g._h, 8, -1), (None, 8, 0)):
+        out = C.c_int()
+        assert sgrid.lib.sg_grid_walk_variant(*bad, C.byref(out)) == sgrid.SG_E_INVALID
+    assert sgrid.lib.sg_grid_walk_variant(g._h, 8, 0, None) == sgrid.SG_E_INVALID
+
+
 def test_explicit_set_matches_restricted_regular():
     """A restricted-regular level set given explicitly gets the identical compact order."""
     from adaptive_sparse_grid_bench_amd import sgrid

@@ -14,6 +14,8 @@ compile-time offsets (n - 1 is a template constant).  From d = 12 (FP64) and d =
     |gpu - oracle| <= 1e-12 * sum_j |term_j|; FP32 within 2e-6 sum_j |term_j| + 1e-7;
   * interpolation at every grid point of the d=10, n=8 (FP32) and d=20, n=6 (FP64) grids;
   * ragged point counts (idle lanes in the last workgroup);
+  * variant 5, the same walk with two points per lane (FP64, depth-first order),
+    bit-identical to variants 4 and 0, and running as variant 4 where it is not built;
   * the staged-pattern probe reads once per subspace per point, each read landing on
     copied surplus data (all-ones surpluses give S times the point's weight).
 """
@@ -93,6 +95,40 @@ def test_blk_equals_variant0_and_oracle(sg, torch_gpu, prec, d, n, P):
         assert (err <= 2e-6 * absref + 1e-7).all(), float(err.max())
 
 
+@pytest.mark.parametrize("d,n", [(9, 6), (10, 8), (11, 5), (9, 3)])
+def test_two_points_per_lane(sg, torch_gpu, d, n):
+    """Variant 5 (two points per lane, FP64, depth-first order) equals variants 4 and 0
+    bit for bit (every point's arithmetic is the lean walk's) on ragged counts around the
+    512 points of a workgroup, and is within the FP64 bound of the oracle."""
+    torch = torch_gpu
+    sh = O.tables(d, n)
+    alpha = O.hierarchize(sh, W.f_gauss(O.grid_coords(sh)) - 0.5 * W.f_reference(O.grid_coords(sh)))
+    g = sg.Grid(d, n)
+    for P in (1, 255, 256, 257, 511, 512, 513, 1500):
+        pts = W.uniform_points(P, d, seed=900 + P)
+        o5 = evaluate(torch, g, alpha, pts, torch.float64, 5)
+        o4 = evaluate(torch, g, alpha, pts, torch.float64, BLK)
+        o0 = evaluate(torch, g, alpha, pts, torch.float64, 0)
+        assert np.array_equal(o5, o4) and np.array_equal(o5, o0), P
+        ref = O.evaluate(sh, alpha, pts)
+        absref = O.evaluate(sh, np.abs(alpha), pts)
+        assert (np.abs(o5 - ref) <= 1e-12 * absref + 1e-300).all()
+
+
+@pytest.mark.parametrize("d,n,prec", [(12, 5, "f64"), (10, 6, "f32"), (6, 4, "f64")])
+def test_two_points_per_lane_falls_back(sg, torch_gpu, d, n, prec):
+    """Where variant 5 is not built (the sparse order, FP32, dimensions without the
+    block-staged walk) it runs as the variant below it: outputs equal variant 4's."""
+    torch = torch_gpu
+    sh = O.tables(d, n)
+    alpha = O.hierarchize(sh, W.f_gauss(O.grid_coords(sh)))
+    g = sg.Grid(d, n)
+    pts = W.uniform_points(700, d, seed=d)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    assert np.array_equal(evaluate(torch, g, alpha, pts, dt, 5),
+                          evaluate(torch, g, alpha, pts, dt, BLK))
+
+
 def test_blk_ragged_counts(sg, torch_gpu):
     torch = torch_gpu
     d, n = 12, 5
@@ -147,7 +183,8 @@ def probe_weight(P, seed):
 
 @pytest.mark.parametrize("d,n,prec,variant", [(12, 4, "f64", BLK), (8, 5, "f32", BLK),
                                               (6, 5, "f64", STAGED), (10, 8, "f64", STAGED),
-                                              (10, 8, "f32", BLK), (20, 6, "f64", BLK)])
+                                              (10, 8, "f32", BLK), (20, 6, "f64", BLK),
+                                              (10, 8, "f64", 5), (9, 4, "f64", 5)])
 def test_stage_probe_reads_once_per_subspace(sg, torch_gpu, d, n, prec, variant):
     """On all-ones surpluses every read of the probe returns 1 exactly when it reads a
     copied surplus (LDS cells of a stage that its copies filled, or a global cell of a

@@ -3,7 +3,8 @@
 //   k_idx2gp / k_gp2idx   the offset <-> (levels, indices) bijection, asg.c:187-243
 //   k_grid_coords         grid-point coordinates (2i+1)/2^(l+1) per offset, asg.c:143-152
 //   k_hierarchize_dim     one dimension's sweep v -= (left + right)/2 over pole families
-//                         held in LDS, asg.c:557-571 semantics
+//                         held in LDS, asg.c:557-571 semantics (k_hierarchize_dim_global:
+//                         the same in global memory for families too large for LDS)
 //   k_eval_dfs            interpolant at one point per lane, depth-first walk over the
 //                         level set with incremental hat products, asg.c:1237-1274 values
 //   k_eval_ref            the same interpolant in the reference's exact operation order
@@ -24,8 +25,8 @@ namespace sg {
 namespace {
 
 constexpr int kThreads = 256;
-
-
+// LDS of a pole-sweep tile (values and family starts); larger tiles sweep in global memory
+constexpr size_t kPoleLdsBytes = 64 * 1024;
 
 #define SG_HIP_CHECK(expr)                                                        \
     do {                                                                          \
@@ -345,6 +346,50 @@ __global__ __launch_bounds__(kThreads) void k_hierarchize_dim(T* __restrict__ v,
             right = buf[fbase + (((1 << pl) - 1) << c) + (hi << (B + pl)) + (pi << B) + lo];
         }
         v[blocks[bfirst + j] + r] = buf[val] - (left + right) * T(0.5);
+    }
+}
+
+// The same sweep for tiles too large for LDS (over kPoleLdsBytes: a family of more than
+// about 8,000 FP64 values, e.g. one dimension reaching level 13 with every other level
+// 0), in place in global memory: one workgroup per tile, its families one after another, each family's blocks from the
+// finest (j = m) to the coarsest (j = 1) with a workgroup barrier between them.  A
+// value's parents in dimension k lie in coarser blocks, which still hold their nodal
+// values when it is updated, so every value gets the LDS sweep's expression on the same
+// operands, bit for bit.
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_hierarchize_dim_global(
+    T* __restrict__ v, const int32_t* __restrict__ tiles, const int32_t* __restrict__ fams,
+    const int32_t* __restrict__ blocks) {
+    const int f0 = tiles[3 * blockIdx.x];
+    const int nf = tiles[3 * blockIdx.x + 1];
+    for (int f = 0; f < nf; ++f) {
+        const int meta = fams[3 * (f0 + f) + 1];
+        const int bfirst = fams[3 * (f0 + f) + 2];
+        const int m = meta & 255, c = (meta >> 8) & 255, B = meta >> 16;
+        for (int j = m; j >= 1; --j) {
+            const int cnt = 1 << (c + j);          // values of block j (at most 2^24)
+            T* const blk = v + blocks[bfirst + j];
+            for (int r = threadIdx.x; r < cnt; r += blockDim.x) {
+                const int lo = r & ((1 << B) - 1);
+                const int ik = (r >> B) & ((1 << j) - 1);
+                const int hi = r >> (B + j);
+                T left = T(0), right = T(0);
+                if (ik != 0) {                      // left parent, asg.c:828-840
+                    const int tz = __builtin_ctz(ik);
+                    const int pl = j - 1 - tz;
+                    const int pi = ik >> (tz + 1);
+                    left = v[blocks[bfirst + pl] + (hi << (B + pl)) + (pi << B) + lo];
+                }
+                if (ik != (1 << j) - 1) {           // right parent, asg.c:842-854
+                    const int tz = __builtin_ctz(ik + 1);
+                    const int pl = j - 1 - tz;
+                    const int pi = (ik + 1) >> (tz + 1);
+                    right = v[blocks[bfirst + pl] + (hi << (B + pl)) + (pi << B) + lo];
+                }
+                blk[r] = blk[r] - (left + right) * T(0.5);
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -823,9 +868,10 @@ int launch_hierarchize_tiles(T* v, const int32_t* tiles, const int32_t* fams,
                              const int32_t* blocks, int num_tiles, int tile_values,
                              int max_families, hipStream_t s) {
     const size_t lds = size_t(tile_values) * sizeof(T) + size_t(max_families) * 4;
-    if (lds > 64 * 1024) {
-        set_error("hierarchize: pole-family tile exceeds 64 KiB of LDS");
-        return kUnsupported;
+    if (lds > kPoleLdsBytes) {
+        hipLaunchKernelGGL(k_hierarchize_dim_global<T>, dim3(num_tiles), dim3(kThreads), 0, s, v,
+                           tiles, fams, blocks);
+        return check_launch("k_hierarchize_dim_global");
     }
     hipLaunchKernelGGL(k_hierarchize_dim<T>, dim3(num_tiles), dim3(kThreads), lds, s, v, tiles,
                        fams, blocks, tile_values);

@@ -172,6 +172,36 @@ def test_hierarchize_tile_sizes(sg, torch_gpu, tile):
     assert (host(t) == O.hierarchize(sh, v)).all()
 
 
+LARGE_POLES = [(1, 15, None), (2, 14, None), (3, 14, [13, 1, 2])]
+
+
+@pytest.mark.parametrize("d,n,lim", LARGE_POLES, ids=ids(LARGE_POLES))
+def test_hierarchize_pole_families_beyond_lds(sg, torch_gpu, d, n, lim):
+    """A dimension reaching level 13 or 14 with the other levels low makes pole families
+    of 16,383-32,767 values, more than a sweep tile holds in LDS: those sweeps run in
+    global memory.  Bit-exact to the oracle in FP64 and FP32, alone and in a batch with
+    an ordinary grid."""
+    torch = torch_gpu
+    sh = O.tables(d, n, lim)
+    v = sample(sh, "mixed")
+    g = sg.Grid(d, n, lim)
+    for dt in (np.float64, np.float32):
+        vv = v.astype(dt)
+        t = dev(torch, vv)
+        g.hierarchize(t)
+        assert (host(t) == O.hierarchize(sh, vv)).all(), dt
+    small = sg.Grid(3, 4)
+    sh_s = O.tables(3, 4)
+    v_s = sample(sh_s, "mixed")
+    b = sg.Batch([small, g])
+    b.set_points([1, 1])
+    t = dev(torch, np.concatenate([v_s, v]))
+    b.hierarchize(t)
+    out = host(t)
+    assert (out[:small.N] == O.hierarchize(sh_s, v_s)).all()
+    assert (out[small.N:] == O.hierarchize(sh, v)).all()
+
+
 # -------------------------------------------------------------------- evaluate --
 
 def check_f64(out, ref, absref):

@@ -253,6 +253,14 @@ struct StateBlk {
     __device__ __forceinline__ void issue(int k) {
         copy_stage<TPB>(alpha, lds, seg_begin, segs, k);
     }
+    // stage 0 landed and stage 1 on its way; called once, before the walk
+    __device__ __forceinline__ void begin() {
+        if (num_stages > 0) {
+            issue(0);
+            stage_barrier();
+            if (num_stages > 1) issue(1);
+        }
+    }
     __device__ __forceinline__ void advance() {
         stage_barrier();
         ++stage;
@@ -1211,11 +1219,7 @@ void k_eval(Params<T> prm) {
         st.base = 0;
         st.cur = 0;
         st.limit = uint32_t(STB);
-        if (st.num_stages > 0) {
-            st.issue(0);
-            stage_barrier();
-            if (st.num_stages > 1) st.issue(1);
-        }
+        st.begin();
     } else if constexpr (STG) {
         __shared__ __attribute__((aligned(16))) uint8_t stage_buf[2 * kStageBytes];
         const cptr<GridDesc::Stages> tb = &G->stage[sizeof(T) == 8 ? 0 : 1];
@@ -1289,11 +1293,7 @@ void k_eval2(Params<T> prm) {
     st.base = 0;
     st.cur = 0;
     st.limit = uint32_t(STB);
-    if (st.num_stages > 0) {
-        st.issue(0);
-        stage_barrier();
-        if (st.num_stages > 1) st.issue(1);
-    }
+    st.begin();
     WalkerBlk2<D - 1, D, J, B, T, St>::run(ln, T(1), T(1), 0u, 0u, 0, NM1, st);
 #pragma unroll
     for (int i = 0; i < 2; ++i)

@@ -57,8 +57,8 @@ int check_device(const sg_grid* hc) {
 }
 
 int finish_create(sg_grid* h, sg_grid** out) {
-    h->tune.variant = sg::default_variant(h->g.d, false);
-    h->tune32.variant = sg::default_variant(h->g.d, true);
+    h->tune.variant = sg::default_variant(h->g.d, h->g.n - 1, false);
+    h->tune32.variant = sg::default_variant(h->g.d, h->g.n - 1, true);
     int st = sg::build_launch_tables(h->g, h->hier_tile_values);
     if (st != sg::kOk) {
         delete h;
@@ -145,7 +145,7 @@ extern "C" {
 
 const char* sg_last_error(void) { return sg::last_error().c_str(); }
 const char* sg_status_string(int status) { return sg::status_name(status); }
-int sg_version(void) { return 204; }
+int sg_version(void) { return 205; }
 
 int sg_grid_create(int num_dims, int num_levels, const int* limits, sg_grid** out) {
     if (!out) return fail(sg::kInvalid, "sg_grid_create: out is NULL");

@@ -38,6 +38,7 @@ namespace sg {
 namespace eval {
 
 constexpr int kThreads = 256;
+static_assert(kThreads == kBlkThreads, "work items fill one block-staged workgroup");
 
 template <class T>
 struct Params {
@@ -329,11 +330,83 @@ __host__ __device__ constexpr int simplex_level() {
     return Simplex<J, R>().l[c][j];
 }
 
+// Lanes that set kPackedUb keep the offset table of tabled dimension 0 only; for the
+// others they keep t_7 (the table's t at level 7) and take t_l = t_7 >> (7 - l): the
+// table's recurrence t_{l+1} = 2 t_l + [m_l >= 0] makes the two equal.
+template <class Ln, class = void>
+struct PackedUb : std::false_type {};
+template <class Ln>
+struct PackedUb<Ln, std::void_t<decltype(Ln::kPackedUb)>>
+    : std::integral_constant<bool, Ln::kPackedUb> {};
+
+// Lanes that set kTopOnTheFly keep no tables for the outermost tabled dimension J - 1:
+// its hat and offset at level l come from x at that level (hat_at), once per run of that
+// level in a static block.
+template <class Ln, class = void>
+struct TopOnTheFly : std::false_type {};
+template <class Ln>
+struct TopOnTheFly<Ln, std::void_t<decltype(Ln::kTopOnTheFly)>>
+    : std::integral_constant<bool, Ln::kTopOnTheFly> {};
+
+template <class T>
+__device__ __forceinline__ void hat_at(T x, int l, T& phi, uint32_t& t);
+
+// whether tabled dimension j of the lane type takes its hats and offsets from x
+template <class Ln, int j>
+__host__ __device__ constexpr bool on_the_fly_dim() {
+    if constexpr (TopOnTheFly<Ln>::value) return j == Ln::kJ - 1;
+    else return false;
+}
+
+// byte offset t_l * sizeof(T) of tabled dimension j at level l
+template <int j, int l, class Ln>
+__device__ __forceinline__ uint32_t ub_at(const Ln& ln) {
+    if constexpr (on_the_fly_dim<Ln, j>()) {
+        std::remove_cv_t<std::remove_reference_t<decltype(ln.x[0])>> phi;
+        uint32_t t;
+        hat_at(ln.x[j], l, phi, t);
+        return t * uint32_t(Ln::kElem);
+    } else if constexpr (PackedUb<Ln>::value && j > 0) {
+        return (ln.t7[j] >> (7 - l)) * uint32_t(Ln::kElem);
+    } else {
+        return ln.ub[j][l];
+    }
+}
+
+// hat of tabled dimension j at level l
+template <int j, int l, class Ln>
+__device__ __forceinline__ auto ph_at(const Ln& ln) {
+    if constexpr (on_the_fly_dim<Ln, j>()) {
+        std::remove_cv_t<std::remove_reference_t<decltype(ln.x[0])>> phi;
+        uint32_t t;
+        hat_at(ln.x[j], l, phi, t);
+        return phi;
+    } else {
+        return ln.ph[j][l];
+    }
+}
+
+// the tables of tabled dimension k of a lane from its coordinate xk
+template <class Ln, class T>
+__device__ __forceinline__ void fill_tables(Ln& ln, int k, T xk) {
+    if constexpr (PackedUb<Ln>::value) {
+        if (k == 0) {
+            level_table<8, T>(xk, ln.ph[0], ln.ub[0]);
+        } else {
+            uint32_t ub[8];
+            level_table<8, T>(xk, ln.ph[k], ub);
+            ln.t7[k] = ub[7] / uint32_t(sizeof(T));
+        }
+    } else {
+        level_table<8, T>(xk, ln.ph[k], ln.ub[k]);
+    }
+}
+
 // in-block byte offset of leaf c: dims J-1 .. 0, each shifted by the levels above it
 template <int J, int R, int c, int j, class Ln>
 __device__ __forceinline__ uint32_t leaf_offset(const Ln& ln, uint32_t off, int sh) {
     constexpr int l = simplex_level<J, R, c, j>();
-    off += ln.ub[j][l] << sh;
+    off += ub_at<j, l>(ln) << sh;
     if constexpr (j > 0) return leaf_offset<J, R, c, j - 1>(ln, off, sh + l);
     return off;
 }
@@ -341,7 +414,7 @@ __device__ __forceinline__ uint32_t leaf_offset(const Ln& ln, uint32_t off, int 
 template <int J, int R, int c, int j, class Ln, class T>
 __device__ __forceinline__ T leaf_weight(const Ln& ln, T w) {
     constexpr int l = simplex_level<J, R, c, j>();
-    w = w * ln.ph[j][l];
+    w = w * ph_at<j, l>(ln);
     if constexpr (j > 0) return leaf_weight<J, R, c, j - 1>(ln, w);
     return w;
 }
@@ -385,8 +458,8 @@ __device__ __forceinline__ void nested_fold(T (&sn)[J], const Ln& ln, T pp, Acc&
         acc.s = fma(pp, sn[J - 1], acc.s);
     } else {
         constexpr int l = simplex_level<J, R, c, j>();
-        if constexpr (l == 0) sn[j] = ln.ph[j][0] * sn[j - 1];
-        else sn[j] = fma(ln.ph[j][l], sn[j - 1], sn[j]);
+        if constexpr (l == 0) sn[j] = ph_at<j, 0>(ln) * sn[j - 1];
+        else sn[j] = fma(ph_at<j, l>(ln), sn[j - 1], sn[j]);
         if constexpr (l == run_max<J, R, c, j>()) nested_fold<J, R, c, j + 1>(sn, ln, pp, acc);
     }
 }
@@ -396,8 +469,8 @@ __device__ __forceinline__ void block_consume_nested(const T (&a)[B], T (&sn)[J]
                                                      T pp, Acc& acc) {
     if constexpr (c < cend) {
         constexpr int l0 = simplex_level<J, R, c, 0>();
-        if constexpr (l0 == 0) sn[0] = ln.ph[0][0] * a[c % B];
-        else sn[0] = fma(ln.ph[0][l0], a[c % B], sn[0]);
+        if constexpr (l0 == 0) sn[0] = ph_at<0, 0>(ln) * a[c % B];
+        else sn[0] = fma(ph_at<0, l0>(ln), a[c % B], sn[0]);
         if constexpr (l0 == run_max<J, R, c, 0>()) nested_fold<J, R, c, 1>(sn, ln, pp, acc);
         block_consume_nested<J, R, B, c + 1, cend>(a, sn, ln, pp, acc);
     }
@@ -425,7 +498,7 @@ __device__ __forceinline__ void block_batches(const Ln& ln, T pp, uint32_t off, 
 template <int J, int R, int c, int j, int SH, class Ln>
 __device__ __forceinline__ uint32_t leaf_offset_ct(const Ln& ln, uint32_t off) {
     constexpr int l = simplex_level<J, R, c, j>();
-    off += ln.ub[j][l] << SH;
+    off += ub_at<j, l>(ln) << SH;
     if constexpr (j > 0) return leaf_offset_ct<J, R, c, j - 1, SH + l>(ln, off);
     return off;
 }
@@ -452,11 +525,20 @@ __device__ __forceinline__ void block_issue_blk(T (&a)[B], const Ln& ln, uint32_
 // offset tables at each block entry keeps that arithmetic inside the block.
 template <int J, int R, class Ln>
 __device__ __forceinline__ void launder_offsets(Ln& ln) {
+    if constexpr (TopOnTheFly<Ln>::value) asm volatile("" : "+v"(ln.x[J - 1]));
+    if constexpr (PackedUb<Ln>::value) {
 #pragma unroll
-    for (int j = 0; j < J; ++j)
+        for (int l = 0; l <= R && l < 8; ++l) asm volatile("" : "+v"(ln.ub[0][l]));
 #pragma unroll
-        for (int l = 0; l <= R && l < int(sizeof(ln.ub[0]) / sizeof(ln.ub[0][0])); ++l)
-            asm volatile("" : "+v"(ln.ub[j][l]));
+        for (int j = 1; j < J - (TopOnTheFly<Ln>::value ? 1 : 0); ++j)
+            asm volatile("" : "+v"(ln.t7[j]));
+    } else {
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+#pragma unroll
+            for (int l = 0; l <= R && l < int(sizeof(ln.ub[0]) / sizeof(ln.ub[0][0])); ++l)
+                asm volatile("" : "+v"(ln.ub[j][l]));
+    }
 }
 
 template <int J, int R, int H, int B, int c0, class Ln, class T, class St>
@@ -697,11 +779,21 @@ __device__ __forceinline__ void hat_at(T x, int l, T& phi, uint32_t& t) {
 
 // Lane state of the block-staged walk: the coordinates and the J tabled dimensions'
 // hat / offset tables.
-template <int D, int J, class T>
+// PACK (the walks at 4 waves per SIMD, blk_quad): the offset table of dimension 0 only,
+// t_7 of the other tabled dimensions (ub_at), and no tables for the outermost tabled
+// dimension, whose hats and offsets come from x once per run (ph_at, ub_at): 162 -> 128
+// VGPRs at d = 10, n = 8.  At 3 waves per SIMD the recomputed hats cost 2 % (146.8 ms
+// against 144.1).
+template <int D, int J, class T, bool PACK = false>
 struct LaneBlk {
+    static constexpr bool kPackedUb = PACK;
+    static constexpr bool kTopOnTheFly = PACK && J > 1;
+    static constexpr int kJ = J;
+    static constexpr int kElem = int(sizeof(T));
     T x[D];
     T ph[J][8];
-    uint32_t ub[J][8];
+    uint32_t ub[PACK ? 1 : J][8];
+    uint32_t t7[J];             // PACK: t_7 of each tabled dimension
 };
 
 // Which walk the block-staged kernels use.  FP64 takes the lean regular walk above
@@ -747,9 +839,8 @@ __device__ __forceinline__ void blk_levels(Ln& ln, T pp, uint32_t off, int hsum,
     }
 }
 
-template <int K, int D, int J, int B, class T, class St>
+template <int K, int D, int J, int B, class T, class St, class Ln = LaneBlk<D, J, T>>
 struct WalkerBlk {
-    using Ln = LaneBlk<D, J, T>;
     static __device__ __forceinline__ void run(Ln& ln, T pp, uint32_t off, int hsum, int rem,
                                                St& st) {
         if constexpr (K == J - 1) {
@@ -762,7 +853,8 @@ struct WalkerBlk {
                 uint32_t t;
                 hat_at(ln.x[K], l, phi, t);
                 const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
-                WalkerBlk<K - 1, D, J, B, T, St>::run(ln, pp * phi, o, hsum + l, rem - l, st);
+                WalkerBlk<K - 1, D, J, B, T, St, Ln>::run(ln, pp * phi, o, hsum + l, rem - l,
+                                                          st);
             }
         }
     }
@@ -1140,7 +1232,8 @@ void k_eval(Params<T> prm) {
     constexpr bool kSparse = NM1 >= 0 && J > 0 && D > J && blk_sparse(D, int(sizeof(T)));
     constexpr bool kLean = NM1 >= 0 && !kSparse && blk_lean_walk<T>();
     using Ln = std::conditional_t<kSparse, LaneSp<(J > 0 ? J : 1), T>,
-                                  std::conditional_t<kLean, LaneBlk<D, J, T>, Lane<D, LT, J, T>>>;
+                                  std::conditional_t<kLean, LaneBlk<D, J, T, (W >= 4)>,
+                                                     Lane<D, LT, J, T>>>;
     Ln ln;
     // sparse walk: coordinates of dimensions J.. and the level-0 hat products Z_c, c =
     // J..D (single_leaf), as 32-bit words (coord_at)
@@ -1152,7 +1245,7 @@ void k_eval(Params<T> prm) {
     for (int k = 0; k < D; ++k) {
         const T xk = x[k];
         if constexpr (J > 0) {
-            if (k < J) level_table<8, T>(xk, ln.ph[k < J ? k : 0], ln.ub[k < J ? k : 0]);
+            if (k < J) fill_tables(ln, k < J ? k : 0, xk);
         }
         if constexpr (kSparse) {
             if (k >= J) {
@@ -1242,7 +1335,7 @@ void k_eval(Params<T> prm) {
         WalkerSp<0, (NM1 > 0 ? NM1 : 0), J, B, T, St, Ln>::run(ln, xlo, xhi, zlo, zhi, D, T(1),
                                                                0u, 0, NM1, st);
     else if constexpr (kLean)
-        WalkerBlk<D - 1, D, J, B, T, St>::run(ln, T(1), 0u, 0, NM1, st);
+        WalkerBlk<D - 1, D, J, B, T, St, Ln>::run(ln, T(1), 0u, 0, NM1, st);
     else
         Walker<D - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
     if (live) prm.out[G->out_off + pc] = st.acc.value();
@@ -1270,7 +1363,7 @@ void k_eval2(Params<T> prm) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const T xk = x[k];
-            if (k < J) level_table<8, T>(xk, ln.p[i].ph[k < J ? k : 0], ln.p[i].ub[k < J ? k : 0]);
+            if (k < J) fill_tables(ln.p[i], k < J ? k : 0, xk);
             ln.p[i].x[k] = xk;
         }
     }
@@ -1406,21 +1499,9 @@ struct Variant {
     int J, B, W;
     bool staged;
 };
-// waves per SIMD the block-staged walk's registers target, per element size (a multiple
-// of the workgroup's waves per SIMD, so whole workgroups fill a CU)
-#ifndef SG_BLK_WAVES64
-#define SG_BLK_WAVES64 3
-#endif
-#ifndef SG_BLK_WAVES32
-#define SG_BLK_WAVES32 3
-#endif
-constexpr int blk_waves(int elem) { return elem == 8 ? SG_BLK_WAVES64 : SG_BLK_WAVES32; }
-static_assert(blk_waves(8) % (blk_threads(8) / 256 > 0 ? blk_threads(8) / 256 : 1) == 0 &&
-              blk_waves(4) % (blk_threads(4) / 256 > 0 ? blk_threads(4) / 256 : 1) == 0,
-              "whole workgroups per CU");
 constexpr Variant kVariants[] = {{3, 8, 3, false}, {2, 8, 3, false}, {3, 4, 4, false},
                                  {kStageJ, 0, 3, true},     // batch: stage_batch()
-                                 {kStageJ, 0, blk_waves(8), true},    // waves: per precision
+                                 {kStageJ, 0, 3, true},     // waves: blk_waves(d, n - 1)
                                  {kStageJ, 0, 2, true}};    // two points per lane
 static_assert(kStagedVariant == 3 && kVariants[kStagedVariant].staged, "staged variant index");
 static_assert(kBlkVariant == 4 && kVariants[kBlkVariant].staged, "block-staged variant index");
@@ -1485,7 +1566,7 @@ hipError_t launch_blk2(int nm1, const Params<T>& prm, int64_t blocks, hipStream_
 #define SG_NM2(NN)                                                                           \
     case NN:                                                                                 \
         hipLaunchKernelGGL((k_eval2<D, Jd, SG_BLK2_BATCH, 2, T, Acc, NN, kBlk2Threads,        \
-                                    blk_stage_bytes(E)>),                                    \
+                                    blk_stage_bytes(D, NN, E)>),                             \
                            dim3(unsigned(blocks)), dim3(kBlk2Threads), 0, s, prm);           \
         return hipGetLastError();
             SG_NM2(2) SG_NM2(3) SG_NM2(4) SG_NM2(5) SG_NM2(6) SG_NM2(7)
@@ -1513,7 +1594,7 @@ hipError_t launch_d_blk2(int d, int nm1, const Params<T>& prm, int64_t blocks, h
 }
 
 // The block-staged walk of a regular grid with n - 1 = nm1 (kBlkMinNM1 .. kBlkMaxNM1),
-// workgroups of blk_threads(sizeof(T)); built for the dimensions blk_built names.
+// workgroups of blk_threads(D, nm1, sizeof(T)); built for the dimensions blk_built names.
 template <int D, class T, class Acc>
 hipError_t launch_blk(int nm1, const Params<T>& prm, int64_t blocks, hipStream_t s)
 #ifdef SG_EVAL_INSTANTIATE
@@ -1524,9 +1605,10 @@ hipError_t launch_blk(int nm1, const Params<T>& prm, int64_t blocks, hipStream_t
         switch (nm1) {
 #define SG_NM(NN)                                                                          \
     case NN:                                                                               \
-        hipLaunchKernelGGL((k_eval<D, 8, Jd, blk_batch<D, T>(), blk_waves(E), false, true, T, \
-                                   Acc, NN, blk_threads(E), blk_stage_bytes(E)>),           \
-                           dim3(unsigned(blocks)), dim3(blk_threads(E)), 0, s, prm);        \
+        hipLaunchKernelGGL((k_eval<D, 8, Jd, blk_batch<D, T>(), blk_waves(D, NN, E), false,  \
+                                   true, T, Acc, NN, blk_threads(D, NN, E),                 \
+                                   blk_stage_bytes(D, NN, E)>),                             \
+                           dim3(unsigned(blocks)), dim3(blk_threads(D, NN, E)), 0, s, prm); \
         return hipGetLastError();
             SG_NM(2) SG_NM(3) SG_NM(4) SG_NM(5) SG_NM(6) SG_NM(7)
 #undef SG_NM
@@ -1599,7 +1681,7 @@ int evaluate_launch(const Grid& g, const DeviceTables& t, const T* alpha, const 
         return kRange;
     }
     const int v = effective_variant(g, tu.variant, int(sizeof(T)));
-    const int block = v == kBlkVariant ? blk_threads(int(sizeof(T)))
+    const int block = v == kBlkVariant ? blk_threads(g.d, g.n - 1, int(sizeof(T)))
                       : kVariants[v].staged ? kThreads : tu.block;
     const int points_per_block = v == kBlk2Variant ? 2 * kBlk2Threads : block;
     const int64_t blocks = (P + points_per_block - 1) / points_per_block;
@@ -1652,18 +1734,31 @@ int launch_stage_probe(const Grid& g, const DeviceTables& t, int variant, const 
     p.P = P;
     p.seed = seed;
     p.out = out;
+    // the block-staged plans' stage size, and the walk's workgroup, follow the shape
+    const bool quad = blk_quad(g.d, g.n - 1, int(sizeof(T)));
     if (v == kBlk2Variant) {
         p.seg_begin = hd.blk[e].seg_begin;
         p.segs = hd.blk[e].segs;
         constexpr int TPB = kBlk2Threads;
-        hipLaunchKernelGGL((k_stage_probe<T, TPB, blk_stage_bytes(int(sizeof(T))), 2>),
-                           dim3(unsigned((P + 2 * TPB - 1) / (2 * TPB))), dim3(TPB), 0, s, p);
+        const dim3 grid(unsigned((P + 2 * TPB - 1) / (2 * TPB)));
+        if (quad)
+            hipLaunchKernelGGL((k_stage_probe<T, TPB, kBlkQuadStageBytes, 2>), grid, dim3(TPB),
+                               0, s, p);
+        else
+            hipLaunchKernelGGL((k_stage_probe<T, TPB, kBlkStageBytes, 2>), grid, dim3(TPB), 0, s,
+                               p);
     } else if (v == kBlkVariant) {
         p.seg_begin = hd.blk[e].seg_begin;
         p.segs = hd.blk[e].segs;
-        constexpr int TPB = blk_threads(int(sizeof(T)));
-        hipLaunchKernelGGL((k_stage_probe<T, TPB, blk_stage_bytes(int(sizeof(T)))>),
-                           dim3(unsigned((P + TPB - 1) / TPB)), dim3(TPB), 0, s, p);
+        if (quad) {
+            constexpr int TPB = kBlkQuadThreads;
+            hipLaunchKernelGGL((k_stage_probe<T, TPB, kBlkQuadStageBytes>),
+                               dim3(unsigned((P + TPB - 1) / TPB)), dim3(TPB), 0, s, p);
+        } else {
+            constexpr int TPB = kBlkThreads;
+            hipLaunchKernelGGL((k_stage_probe<T, TPB, kBlkStageBytes>),
+                               dim3(unsigned((P + TPB - 1) / TPB)), dim3(TPB), 0, s, p);
+        }
     } else {
         p.seg_begin = hd.stage[e].seg_begin;
         p.segs = hd.stage[e].segs;

@@ -689,8 +689,8 @@ int build_launch_tables(Grid& g, int hier_tile_values) {
                 build_stage_plan(g, J, blocks, 8, g.stage[0]);
                 build_stage_plan(g, J, blocks, 4, g.stage[1]);
                 if (J == kStageJ) {
-                    build_blk_plan(g, J, blocks, 8, blk_stage_bytes(8), g.blk[0]);
-                    build_blk_plan(g, J, blocks, 4, blk_stage_bytes(4), g.blk[1]);
+                    build_blk_plan(g, J, blocks, 8, blk_stage_bytes(d, g.n - 1, 8), g.blk[0]);
+                    build_blk_plan(g, J, blocks, 4, blk_stage_bytes(d, g.n - 1, 4), g.blk[1]);
                 }
             }
         }

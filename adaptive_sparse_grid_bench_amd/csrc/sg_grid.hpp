@@ -80,27 +80,9 @@ SG_HOST_DEVICE constexpr int stage_hmax(int J, int R, int elem) {
 // leaves of level sum s (relative to the block) form one run, the runs s = 0..R follow
 // each other, each padded to 16 bytes, and inside a run the leaves keep their HBM order
 // (l_{J-1}, ..., l_1 ascending, the compact order of one level sum).  Every run is one
-// contiguous range in HBM as well.  A workgroup of blk_threads(elem) threads
-// double-buffers stages of blk_stage_bytes(elem); a stage takes the next blocks in walk
-// order while they fit.
-#ifndef SG_BLK_THREADS64
-#define SG_BLK_THREADS64 256
-#endif
-#ifndef SG_BLK_STAGE_BYTES64
-#define SG_BLK_STAGE_BYTES64 26624
-#endif
-#ifndef SG_BLK_THREADS32
-#define SG_BLK_THREADS32 256
-#endif
-#ifndef SG_BLK_STAGE_BYTES32
-#define SG_BLK_STAGE_BYTES32 26624
-#endif
-SG_HOST_DEVICE constexpr int blk_threads(int elem) {
-    return elem == 8 ? SG_BLK_THREADS64 : SG_BLK_THREADS32;
-}
-SG_HOST_DEVICE constexpr int blk_stage_bytes(int elem) {
-    return elem == 8 ? SG_BLK_STAGE_BYTES64 : SG_BLK_STAGE_BYTES32;
-}
+// contiguous range in HBM as well.  A workgroup of blk_threads(d, n - 1, elem) threads
+// double-buffers stages of blk_stage_bytes(d, n - 1, elem); a stage takes the next
+// blocks in walk order while they fit.
 // the block-staged kernels are built for n - 1 in kBlkMinNM1..kBlkMaxNM1 (the tabled
 // dimensions hold levels 0..7) and for the dimensions where they beat the staged walk
 // (2^22-2^24 points, n = 5..8: FP64 from d = 12 on, up to 1.5x at d = 16..20 in the
@@ -195,6 +177,30 @@ SG_HOST_DEVICE constexpr bool blk_sparse(int d, int elem) {
 //   at d = 10) do not pay for that.  The autotuner keeps it as a candidate.
 SG_HOST_DEVICE constexpr bool blk2_built(int d, int elem) {
     return elem == 8 && blk_built(d, elem) && !blk_sparse(d, elem);
+}
+
+// Workgroup shape of the block-staged walk.  The FP64 depth-first walk of n = 8 grids at
+// d = 9..11 keeps the offset table of tabled dimension 0 only and computes the hats of
+// the outermost tabled dimension once per run (LaneBlk with kPackedUb), which fits it in
+// 128 VGPRs: 512-thread workgroups at 4 waves per SIMD, two per CU, each double-buffering
+// stages of 40448 bytes, so one copied stage serves 512 points.  d = 10, n = 8, 2^24
+// points: 144.1 ms at 256 threads and 3 waves per SIMD (26 KiB stages), 135.8 ms at 256
+// threads and 4 waves (19.75 KiB stages), 126.9 ms as here.  With n < 8 the same walk
+// spills at 128 VGPRs.  Every other block-staged walk: 256 threads, 3 waves per SIMD,
+// 26 KiB stages.
+constexpr int kBlkThreads = 256, kBlkStageBytes = 26624, kBlkWaves = 3;
+constexpr int kBlkQuadThreads = 512, kBlkQuadStageBytes = 40448, kBlkQuadWaves = 4;
+SG_HOST_DEVICE constexpr bool blk_quad(int d, int nm1, int elem) {
+    return elem == 8 && nm1 == kBlkMaxNM1 && d >= 9 && d <= 11 && !blk_sparse(d, elem);
+}
+SG_HOST_DEVICE constexpr int blk_threads(int d, int nm1, int elem) {
+    return blk_quad(d, nm1, elem) ? kBlkQuadThreads : kBlkThreads;
+}
+SG_HOST_DEVICE constexpr int blk_stage_bytes(int d, int nm1, int elem) {
+    return blk_quad(d, nm1, elem) ? kBlkQuadStageBytes : kBlkStageBytes;
+}
+SG_HOST_DEVICE constexpr int blk_waves(int d, int nm1, int elem) {
+    return blk_quad(d, nm1, elem) ? kBlkQuadWaves : kBlkWaves;
 }
 
 // How the block-staged walk copies one grid's blocks into LDS, for one element size.

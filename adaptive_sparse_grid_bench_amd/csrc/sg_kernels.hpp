@@ -137,12 +137,15 @@ constexpr int kBlkVariant = 4;
 // the block-staged walk with two points per lane (blk2_built: FP64, depth-first order)
 constexpr int kBlk2Variant = 5;
 
-// default variants by dimension and precision, from timings at 2^22-2^24 points on
-// d = 5..16 and 20 (the autotuner refines them per grid shape); a grid the block-staged
-// walk does not cover runs the staged walk
-inline int default_variant(int d, bool f32) {
+// default variants by dimension, n - 1 and precision, from timings at 2^22-2^24 points
+// on d = 5..16 and 20 (the autotuner refines them per grid shape); a grid the
+// block-staged walk does not cover runs the staged walk.  FP64 below d = 12 takes the
+// block-staged walk where it runs at 4 waves per SIMD (blk_quad: n = 8, d = 9..11;
+// 2^22 points at d = 9 and 11, 2^24 at d = 10: 16.7 / 127.0 / 58.7 ms against the staged
+// walk's 19.4 / 143.9 / 65.9 ms); at 3 waves per SIMD it ties with the staged walk there.
+inline int default_variant(int d, int nm1, bool f32) {
     if (f32) return d >= 8 ? kBlkVariant : kStagedVariant;
-    return d >= 12 ? kBlkVariant : kStagedVariant;   // falls back to the staged walk
+    return d >= 12 || blk_quad(d, nm1, 8) ? kBlkVariant : kStagedVariant;
 }
 // the variant a launch runs: the staged walk falls back to variant 0 for a grid whose
 // blocks cannot be staged in this element size
@@ -169,7 +172,7 @@ inline int work_list_variant(const Grid& g, int variant, int elem_bytes) {
     // work items hold one point per lane: the two-point walk runs as the block-staged one
     const int v = effective_variant(g, variant == kBlk2Variant ? kBlkVariant : variant,
                                     elem_bytes);
-    if (v == kBlkVariant && blk_threads(elem_bytes) != 256)
+    if (v == kBlkVariant && blk_threads(g.d, g.n - 1, elem_bytes) != kBlkThreads)
         return effective_variant(g, kStagedVariant, elem_bytes);
     return v;
 }

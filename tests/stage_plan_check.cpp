@@ -121,7 +121,7 @@ int check_blk_plan(const sg::Grid& g, int e) {
         std::printf("d=%d n=%d elem=%d: no block plan\n", g.d, g.n, elem);
         return 0;
     }
-    const int J = std::min(g.d, sg::kStageJ), nm1 = g.n - 1, STB = sg::blk_stage_bytes(elem);
+    const int J = std::min(g.d, sg::kStageJ), nm1 = g.n - 1, STB = sg::blk_stage_bytes(g.d, g.n - 1, elem);
     const int K = p.num_stages;
     if (int(p.seg_begin.size()) != 2 * K + 1) return fail("blk seg_begin has 2K+1 entries", g.d, elem, -1);
     if (int(p.segs.size()) != 2 * (p.seg_begin[2 * K] + 4))

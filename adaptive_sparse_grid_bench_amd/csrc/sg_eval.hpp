@@ -289,13 +289,20 @@ struct StateBlk {
 };
 
 // Per-lane state: m0 = 2x - 1 per dimension for the incremental walk, and the hat /
-// offset tables of the J innermost dimensions.
-template <int D, int LT, int J, class T>
+// offset tables of the J innermost dimensions.  PACK (the FP32 block-staged walk at 4
+// waves per SIMD, blk_quad): the packed tables of LaneBlk (ub_at, ph_at); 128 VGPRs and
+// 44 bytes of scratch per lane at d = 10, n = 8.
+template <int D, int LT, int J, class T, bool PACK = false>
 struct Lane {
+    static constexpr bool kPackedUb = PACK;
+    static constexpr bool kTopOnTheFly = PACK && J > 1;
     static constexpr int kJ = J;
+    static constexpr int kElem = int(sizeof(T));
     T m0[D];
     T ph[J > 0 ? J : 1][J > 0 ? LT : 1];
-    uint32_t ub[J > 0 ? J : 1][J > 0 ? LT : 1];
+    uint32_t ub[J > 0 && !PACK ? J : 1][J > 0 ? LT : 1];
+    uint32_t t7[J > 0 ? J : 1];     // PACK: t_7 of each tabled dimension
+    T x[J > 0 ? J : 1];             // PACK: the outermost tabled dimension's coordinate
 };
 
 // The level vectors (l_{J-1}, ..., l_0) with sum <= R in walk order (dim J-1
@@ -1150,9 +1157,9 @@ struct WalkerSp {
 // Node codes of an explicit set (one byte per visited node, walk order): at depth J-1
 // a code below 0x80 is a full static block of budget R = code; otherwise code & 0x7f is
 // the largest l_K under this node.
-template <int K, int D, int LT, int J, int B, bool EXPL, class T, class St>
+template <int K, int D, int LT, int J, int B, bool EXPL, class T, class St,
+          class Ln = Lane<D, LT, J, T>>
 struct Walker {
-    using Ln = Lane<D, LT, J, T>;
     static __device__ __forceinline__ void run(const Ln& ln, const int* lim, T pp, uint32_t off,
                                                int hsum, int rem, St& st) {
         int b;
@@ -1204,7 +1211,7 @@ struct Walker {
             const T phi = T(1) - fabs(m);
             const uint32_t o = off + ((t * uint32_t(sizeof(T))) << hsum);
             if constexpr (K == 0) st.leaf(pp * phi, o);
-            else Walker<K - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, pp * phi, o, hsum + l,
+            else Walker<K - 1, D, LT, J, B, EXPL, T, St, Ln>::run(ln, lim, pp * phi, o, hsum + l,
                                                                rem - l, st);
             const bool ge = m >= T(0);
             t = 2u * t + (ge ? 1u : 0u);
@@ -1243,7 +1250,7 @@ void k_eval(Params<T> prm) {
     constexpr bool kLean = NM1 >= 0 && !kSparse && blk_lean_walk<T>();
     using Ln = std::conditional_t<kSparse, LaneSp<(J > 0 ? J : 1), T>,
                                   std::conditional_t<kLean, LaneBlk<D, J, T, (W >= 4)>,
-                                                     Lane<D, LT, J, T>>>;
+                                                     Lane<D, LT, J, T, (NM1 >= 0 && W >= 4)>>>;
     Ln ln;
     // sparse walk: coordinates of dimensions J.. and the level-0 hat products Z_c, c =
     // J..D (single_leaf), as 32-bit words (coord_at)
@@ -1273,6 +1280,8 @@ void k_eval(Params<T> prm) {
             ln.x[k] = xk;
         } else {
             ln.m0[k] = T(2) * xk - T(1);
+            if constexpr (TopOnTheFly<Ln>::value)
+                if (k == J - 1) ln.x[J - 1 >= 0 ? J - 1 : 0] = xk;
         }
     }
     if constexpr (kSparse) {
@@ -1349,7 +1358,7 @@ void k_eval(Params<T> prm) {
     else if constexpr (kLean)
         WalkerBlk<D - 1, D, J, B, T, St, Ln>::run(ln, T(1), 0u, 0, NM1, st);
     else
-        Walker<D - 1, D, LT, J, B, EXPL, T, St>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
+        Walker<D - 1, D, LT, J, B, EXPL, T, St, Ln>::run(ln, lim, T(1), 0u, 0, G->n - 1, st);
     if (live) prm.out[G->out_off + pc] = st.acc.value();
 }
 

@@ -3,9 +3,9 @@
 Variant 4 runs on regular grids with n - 1 in 2..7, FP64 for d = 9..20 and FP32 for
 d = 7..20 (other grids run variant 3).  Its static blocks sit in LDS in walk order at
 compile-time offsets (n - 1 is a template constant).  From d = 12 (FP64) and d = 15
-(FP32) on it walks the sparse order (sgrid.blk_sparse).  FP64 grids with n = 8 at
-d = 9..11 run it in 512-thread workgroups at 4 waves per SIMD with packed lane tables
-(d9n8, d10n8, d11n8 below).  Bars:
+(FP32) on it walks the sparse order (sgrid.blk_sparse).  Grids with n = 8 at d = 9..11
+run it in 512-thread workgroups at 4 waves per SIMD with packed lane tables (d9n8, d10n8,
+d11n8 below, both precisions).  Bars:
   * depth-first order: outputs bit-identical to variant 0 (same leaves, same order, same
     arithmetic per point; the FP64 walk computes each outer hat from x at its level,
     which on the 2^-24 lattice equals the incremental form);
@@ -58,7 +58,8 @@ def evaluate(torch, g, alpha, pts, dt, variant):
 SHAPES64 = [(9, 6, 700), (9, 8, 700), (10, 8, 1100), (11, 3, 900), (11, 8, 300), (12, 5, 700),
             (14, 4, 800),
             (16, 6, 200), (17, 4, 500), (20, 3, 1000), (20, 5, 300), (20, 8, 40)]
-SHAPES32 = [(7, 8, 900), (8, 3, 1000), (10, 8, 500), (13, 6, 300), (15, 5, 400), (16, 4, 700),
+SHAPES32 = [(7, 8, 900), (8, 3, 1000), (9, 8, 1100), (10, 8, 500), (11, 8, 300), (13, 6, 300),
+            (15, 5, 400), (16, 4, 700),
             (18, 5, 300), (20, 6, 100)]
 
 
@@ -188,7 +189,7 @@ def probe_weight(P, seed):
                                               (6, 5, "f64", STAGED), (10, 8, "f64", STAGED),
                                               (10, 8, "f32", BLK), (20, 6, "f64", BLK),
                                               (10, 8, "f64", 5), (9, 4, "f64", 5),
-                                              (10, 8, "f64", BLK)])
+                                              (10, 8, "f64", BLK), (9, 8, "f32", BLK)])
 def test_stage_probe_reads_once_per_subspace(sg, torch_gpu, d, n, prec, variant):
     """On all-ones surpluses every read of the probe returns 1 exactly when it reads a
     copied surplus (LDS cells of a stage that its copies filled, or a global cell of a

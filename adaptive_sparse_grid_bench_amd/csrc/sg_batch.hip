@@ -196,12 +196,19 @@ int Batch::set_points(const std::vector<int64_t>& counts) {
         D.out_off = point_off[i];
         // table length valid for every variant (variant 0 tables the most dimensions)
         const int lt = eval::table_levels(g, 0);
-        const int v64 = work_list_variant(g, grids[i].variant_f64, 8);
-        const int v32 = work_list_variant(g, grids[i].variant_f32, 4);
+        int v64 = work_list_variant(g, grids[i].variant_f64, 8);
+        int v32 = work_list_variant(g, grids[i].variant_f32, 4);
+        // both precisions share the group's work items, which hold as many points as the
+        // walks' workgroups: where those differ (one precision at 4 waves per SIMD, 512
+        // threads) that precision runs the staged walk instead
+        const int p64 = work_item_points(g, v64, 8), p32 = work_item_points(g, v32, 4);
+        if (p64 > p32) v64 = effective_variant(g, kStagedVariant, 8);
+        if (p32 > p64) v32 = effective_variant(g, kStagedVariant, 4);
+        const int64_t per_item = std::min(p64, p32);
         const int nm1 = (v64 == kBlkVariant || v32 == kBlkVariant) ? g.n - 1 : -1;
         auto& items = groups_h[std::make_tuple(g.explicit_set ? 1 : 0, g.d, lt, v64, v32, nm1)];
-        for (int64_t f = 0; f < counts[i]; f += eval::kThreads)
-            items.push_back({i, int32_t(f), int32_t(std::min<int64_t>(eval::kThreads, counts[i] - f)), 0});
+        for (int64_t f = 0; f < counts[i]; f += per_item)
+            items.push_back({i, int32_t(f), int32_t(std::min<int64_t>(per_item, counts[i] - f)), 0});
     }
     std::vector<eval::WorkItem> all;
     groups.clear();

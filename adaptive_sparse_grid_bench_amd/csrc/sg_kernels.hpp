@@ -165,16 +165,15 @@ inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
     if (variant == kStagedVariant && !g.stage[e].ok) return 0;
     return variant;
 }
-// the variant of a work-list launch (batches, the server): work items hold up to
-// eval::kThreads points, the workgroup size of every walk there; a block-staged grid
-// whose workgroups differ in size runs the staged walk instead
+// the variant of a work-list launch (batches, the server); work items hold one point per
+// lane, so the two-point walk runs as the block-staged one
 inline int work_list_variant(const Grid& g, int variant, int elem_bytes) {
-    // work items hold one point per lane: the two-point walk runs as the block-staged one
-    const int v = effective_variant(g, variant == kBlk2Variant ? kBlkVariant : variant,
-                                    elem_bytes);
-    if (v == kBlkVariant && blk_threads(g.d, g.n - 1, elem_bytes) != kBlkThreads)
-        return effective_variant(g, kStagedVariant, elem_bytes);
-    return v;
+    return effective_variant(g, variant == kBlk2Variant ? kBlkVariant : variant, elem_bytes);
+}
+// points per work item of a walk in a work list: its workgroup size (512 for the
+// block-staged walk at 4 waves per SIMD, blk_quad; 256 for every other walk)
+inline int work_item_points(const Grid& g, int variant, int elem_bytes) {
+    return variant == kBlkVariant ? blk_threads(g.d, g.n - 1, elem_bytes) : kBlkThreads;
 }
 
 // perm: lane order of the points (from sort_points) or null for the given order

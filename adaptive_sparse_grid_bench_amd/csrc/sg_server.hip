@@ -229,9 +229,10 @@ struct Server::Impl {
                                              ge.variant == kBlkVariant ? g.n - 1 : -1);
             auto& items = groups[key];
             group_reqs[key].push_back(i);
-            for (int64_t f = 0; f < r.P; f += eval::kThreads)
+            const int64_t per_item = work_item_points(g, ge.variant, int(elem));
+            for (int64_t f = 0; f < r.P; f += per_item)
                 items.push_back({int32_t(i), int32_t(f),
-                                 int32_t(std::min<int64_t>(eval::kThreads, r.P - f)), 0});
+                                 int32_t(std::min<int64_t>(per_item, r.P - f)), 0});
             out_off[i] = ooff;
             coff += r.P * g.d;
             ooff += r.P;

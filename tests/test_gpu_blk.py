@@ -214,17 +214,20 @@ def test_stage_probe_reads_once_per_subspace(sg, torch_gpu, d, n, prec, variant)
 
 def test_blk_in_batches_and_the_server(sg, torch_gpu):
     """Work-list launches (batches, their graph, the server) run the block-staged walk
-    for the grids tuned to it, grouped by n; outputs equal the per-grid calls."""
+    for the grids tuned to it, grouped by n; outputs equal the per-grid calls.  The
+    n = 8 grids at d = 9..11 run it in 512-thread workgroups with work items of 512
+    points (ragged counts around that size); the d = 11 one is tuned to a gathering walk
+    in FP32, so in a batch its FP64 walk falls back to the staged one."""
     torch = torch_gpu
-    shapes = [(12, 5), (11, 4), (9, 6), (12, 6), (8, 5), (3, 4)]
+    shapes = [(12, 5), (11, 4), (9, 6), (12, 6), (8, 5), (3, 4), (10, 8), (9, 8), (11, 8)]
     grids, vals = [], []
     for d, n in shapes:
         sh = O.tables(d, n)
         g = sg.Grid(d, n)
-        g.set_tuning(eval_variant_f64=BLK, eval_variant_f32=BLK)
+        g.set_tuning(eval_variant_f64=BLK, eval_variant_f32=2 if (d, n) == (11, 8) else BLK)
         grids.append(g)
         vals.append(W.f_gauss(O.grid_coords(sh)))
-    counts = [300, 1000, 257, 64, 700, 5]
+    counts = [300, 1000, 257, 64, 700, 5, 1100, 513, 900]
     b = sg.Batch(grids)
     b.set_points(counts)
     nodal = torch.from_numpy(np.concatenate(vals)).cuda()

@@ -140,7 +140,7 @@ constexpr int kBlk2Variant = 5;
 // default variants by dimension, n - 1 and precision, from timings at 2^22-2^24 points
 // on d = 5..16 and 20 (the autotuner refines them per grid shape); a grid the
 // block-staged walk does not cover runs the staged walk.  FP64 below d = 12 takes the
-// block-staged walk where it runs at 4 waves per SIMD (blk_quad: n = 7 and 8, d = 9..11;
+// block-staged walk where it runs at 4 waves per SIMD (blk_quad: n = 8 at d = 9..11, n = 7 at d = 10, 11;
 // 2^22 points at d = 9 and 11, 2^24 at d = 10: 16.7 / 127.0 / 58.7 ms against the staged
 // walk's 19.4 / 143.9 / 65.9 ms); at 3 waves per SIMD it ties with the staged walk there.
 inline int default_variant(int d, int nm1, bool f32) {

@@ -4,8 +4,8 @@ Variant 4 runs on regular grids with n - 1 in 2..7, FP64 for d = 9..20 and FP32 
 d = 7..20 (other grids run variant 3).  Its static blocks sit in LDS in walk order at
 compile-time offsets (n - 1 is a template constant).  From d = 12 (FP64) and d = 15
 (FP32) on it walks the sparse order (sgrid.blk_sparse).  Grids with n = 8 at d = 9..11
-(FP64: n = 7 and 8) run it in 512-thread workgroups at 4 waves per SIMD with packed lane
-tables (d9n8, d10n7, d10n8, d11n8 below).  Bars:
+(FP64 also n = 7 at d = 10, 11) run it in 512-thread workgroups at 4 waves per SIMD with
+packed lane tables (d9n8, d10n7, d10n8, d11n8 below).  Bars:
   * depth-first order: outputs bit-identical to variant 0 (same leaves, same order, same
     arithmetic per point; the FP64 walk computes each outer hat from x at its level,
     which on the 2^-24 lattice equals the incremental form);

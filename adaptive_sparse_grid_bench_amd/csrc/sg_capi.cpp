@@ -145,7 +145,7 @@ extern "C" {
 
 const char* sg_last_error(void) { return sg::last_error().c_str(); }
 const char* sg_status_string(int status) { return sg::status_name(status); }
-int sg_version(void) { return 209; }
+int sg_version(void) { return 210; }
 
 int sg_grid_create(int num_dims, int num_levels, const int* limits, sg_grid** out) {
     if (!out) return fail(sg::kInvalid, "sg_grid_create: out is NULL");

@@ -180,22 +180,23 @@ SG_HOST_DEVICE constexpr bool blk2_built(int d, int elem) {
 }
 
 // Workgroup shape of the block-staged walk.  The depth-first walk of n = 8 grids at
-// d = 9..11 (FP64: and n = 7 at d = 10, 11) keeps the offset table of tabled dimension 0 only and computes the hats of
+// d = 9..11 and n = 7 grids at d = 10, 11 keeps the offset table of tabled dimension 0 only and computes the hats of
 // the outermost tabled dimension once per run (packed lane tables), which fits it in 128
 // VGPRs: 512-thread workgroups at 4 waves per SIMD, two per CU, each double-buffering
 // stages of 40448 bytes, so one copied stage serves 512 points.  d = 10, n = 8, 2^24
 // points, FP64: 144.1 ms at 256 threads and 3 waves per SIMD (26 KiB stages), 135.8 ms
 // at 256 threads and 4 waves (19.75 KiB stages), 126.9 ms as here (one workgroup of
 // 1024 threads per CU with 80896-byte stages: 147.2 ms); FP32-Kahan: 99.4 -> 85.4 ms.
-// FP64 n = 7 grids spill 48-64 bytes per lane at 128 VGPRs and still gain at d = 10 and
-// 11 (2^22 points: 16.4 -> 15.5 ms, 29.7 -> 27.6 ms), not at d = 9 (8.24 ms against the
-// staged walk's 7.48); at n <= 6 it is even (d = 10, n = 6: 6.80 / 6.83 ms).  Every other block-staged walk: 256 threads, 3 waves per SIMD,
+// n = 7 grids spill 44-64 bytes per lane at 128 VGPRs and still gain at d = 10 and 11
+// (2^22 points, FP64: 16.4 -> 15.5 ms, 29.7 -> 27.6 ms; FP32-Kahan: 10.4 -> 9.7 ms,
+// 17.6 -> 16.6 ms), not at d = 9 (FP64 8.24 ms against the staged walk's 7.48); at
+// n <= 6 it is even (d = 10, n = 6 FP64: 6.80 / 6.83 ms).  Every other block-staged walk: 256 threads, 3 waves per SIMD,
 // 26 KiB stages.
 constexpr int kBlkThreads = 256, kBlkStageBytes = 26624, kBlkWaves = 3;
 constexpr int kBlkQuadThreads = 512, kBlkQuadStageBytes = 40448, kBlkQuadWaves = 4;
 SG_HOST_DEVICE constexpr bool blk_quad(int d, int nm1, int elem) {
     return d >= 9 && d <= 11 && !blk_sparse(d, elem) && nm1 <= kBlkMaxNM1 &&
-           nm1 >= (elem == 8 && d >= 10 ? kBlkMaxNM1 - 1 : kBlkMaxNM1);
+           nm1 >= (d >= 10 ? kBlkMaxNM1 - 1 : kBlkMaxNM1);
 }
 SG_HOST_DEVICE constexpr int blk_threads(int d, int nm1, int elem) {
     return blk_quad(d, nm1, elem) ? kBlkQuadThreads : kBlkThreads;

@@ -4,8 +4,8 @@ Variant 4 runs on regular grids with n - 1 in 2..7, FP64 for d = 9..20 and FP32 
 d = 7..20 (other grids run variant 3).  Its static blocks sit in LDS in walk order at
 compile-time offsets (n - 1 is a template constant).  From d = 12 (FP64) and d = 15
 (FP32) on it walks the sparse order (sgrid.blk_sparse).  Grids with n = 8 at d = 9..11
-(FP64 also n = 7 at d = 10, 11) run it in 512-thread workgroups at 4 waves per SIMD with
-packed lane tables (d9n8, d10n7, d10n8, d11n8 below).  Bars:
+and n = 7 at d = 10, 11 run it in 512-thread workgroups at 4 waves per SIMD with packed
+lane tables (d9n8, d10n7, d10n8, d11n8 below).  Bars:
   * depth-first order: outputs bit-identical to variant 0 (same leaves, same order, same
     arithmetic per point; the FP64 walk computes each outer hat from x at its level,
     which on the 2^-24 lattice equals the incremental form);
@@ -59,7 +59,8 @@ SHAPES64 = [(9, 6, 700), (9, 8, 700), (10, 7, 600), (10, 8, 1100), (11, 3, 900),
             (12, 5, 700),
             (14, 4, 800),
             (16, 6, 200), (17, 4, 500), (20, 3, 1000), (20, 5, 300), (20, 8, 40)]
-SHAPES32 = [(7, 8, 900), (8, 3, 1000), (9, 8, 1100), (10, 8, 500), (11, 8, 300), (13, 6, 300),
+SHAPES32 = [(7, 8, 900), (8, 3, 1000), (9, 8, 1100), (10, 8, 500), (11, 7, 600), (11, 8, 300),
+            (13, 6, 300),
             (15, 5, 400), (16, 4, 700),
             (18, 5, 300), (20, 6, 100)]
 

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "sg_grid.hpp"
@@ -151,6 +152,15 @@ inline int default_variant(int d, int nm1, bool f32) {
 // blocks cannot be staged in this element size
 inline int effective_variant(const Grid& g, int variant, int elem_bytes) {
     const int e = elem_bytes == 8 ? 0 : 1;
+    // the gathering walks 1 and 2 differ from 0 only in their tabled dimensions and batch:
+    // explicit sets, and grids whose tabled dimensions reach level 8, run the one
+    // table-free gathering walk (sg_eval.hpp launch_variant / launch_d)
+    if (variant == 1 || variant == 2) {
+        bool tabled = !g.explicit_set;
+        for (int j = 0; j < std::min(g.d, variant == 1 ? 2 : 3) && tabled; ++j)
+            tabled = g.limits[j] <= 7;
+        if (!tabled) return 0;
+    }
     if (variant == kBlk2Variant) {
         if (!blk2_built(g.d, elem_bytes)) variant = kBlkVariant;
         else if (g.explicit_set || !g.blk[e].ok || g.n - 1 < kBlkMinNM1 || g.n - 1 > kBlkMaxNM1)

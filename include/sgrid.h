@@ -79,7 +79,9 @@ int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *bas
  * with other calls on the same grid).  The autotuner (package module autotune.py)
  * times the candidates on the GPU and stores its choice per grid shape.
  *   eval_block        threads per workgroup of the evaluate kernels: 64, 128 or 256
- *                     (the staged variant always runs 256);
+ *                     (the staged walks choose their own: 256, and 512 for the
+ *                     block-staged walk of n = 8 grids at d = 9..11 and n = 7 grids at
+ *                     d = 10, 11, which runs at 4 waves per SIMD);
  *   eval_variant_f64  walk variant of the FP64 evaluate,
  *   eval_variant_f32  and of the FP32 one: 0..SG_EVAL_VARIANTS-1.  0..2 gather each
  *                     coefficient per lane from HBM/cache (they differ in tabled
@@ -91,7 +93,8 @@ int sg_grid_subspaces(const sg_grid *grid, unsigned char *levels, long long *bas
  *                     d = 12 in FP64 and d = 15 in FP32); 5 is the block-staged walk with
  *                     two query points per lane (FP64 below d = 12; elsewhere it runs as
  *                     4).  A grid a variant does not cover runs the next more general
- *                     one (5 -> 4 -> 3 -> 0);
+ *                     one (5 -> 4 -> 3 -> 0; 1 and 2 -> 0 on explicit sets and where
+ *                     their tabled dimensions reach level 8);
  *   bin_min_points    evaluate launches with at least this many points bin the points
  *                     spatially first (<= 0: never; the staged variants never bin);
  *   hier_tile_values  target values per hierarchization workgroup tile (>= 64). */
